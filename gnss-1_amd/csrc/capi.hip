@@ -1,0 +1,227 @@
+// capi.hip — the extern "C" boundary of include/gnss.h for the MI355X engine.
+//
+// Every entry point catches engine exceptions and turns them into a status code plus a message
+// retrievable with gnss_last_error (the reference returns void and aborts or prints,
+// fftfuncs.cu:297-300; allocate.cu:109-142).
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "engine.h"
+
+#define GNSS_API extern "C" __attribute__((visibility("default")))
+
+using gnss::EngineBase;
+using gnss::Error;
+
+struct gnss_ctx {
+  EngineBase* eng = nullptr;
+  gnss_params p{};
+  std::string err;
+};
+
+static thread_local std::string g_create_err;
+
+namespace gnss {
+
+// ---------------------------------------------------------------------------------------------
+// KernelTimer
+// ---------------------------------------------------------------------------------------------
+void KernelTimer::enable(bool on, hipStream_t s) {
+  drain();
+  on_ = on;
+  stream_ = s;
+  if (on) agg_.clear();
+}
+
+hipEvent_t KernelTimer::get_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  GNSS_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void KernelTimer::begin(const char* name, double bytes) {
+  Rec r;
+  r.name = name;
+  r.bytes = bytes;
+  r.a = get_event();
+  r.b = get_event();
+  GNSS_HIP(hipEventRecord(r.a, stream_));
+  pending_.push_back(r);
+  if (pending_.size() > 4096) drain();
+}
+
+void KernelTimer::end() { GNSS_HIP(hipEventRecord(pending_.back().b, stream_)); }
+
+void KernelTimer::drain() {
+  if (pending_.empty()) return;
+  GNSS_HIP(hipEventSynchronize(pending_.back().b));
+  for (Rec& r : pending_) {
+    float ms = 0;
+    GNSS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    Agg& a = agg_[r.name];
+    a.launches += 1;
+    a.ms += ms;
+    a.bytes += r.bytes;
+    pool_.push_back(r.a);
+    pool_.push_back(r.b);
+  }
+  pending_.clear();
+}
+
+std::string KernelTimer::report() {
+  drain();
+  std::string s = "{";
+  bool first = true;
+  char buf[256];
+  for (auto& kv : agg_) {
+    snprintf(buf, sizeof buf, "%s\"%s\": {\"launches\": %lld, \"ms\": %.6f, \"bytes\": %.0f}", first ? "" : ", ",
+             kv.first.c_str(), kv.second.launches, kv.second.ms, kv.second.bytes);
+    s += buf;
+    first = false;
+  }
+  return s + "}";
+}
+
+KernelTimer::~KernelTimer() {
+  for (Rec& r : pending_) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+}  // namespace gnss
+
+template <typename Fn> static int guard(gnss_ctx* c, Fn&& fn) {
+  if (!c || !c->eng) return GNSS_ERR_ARG;
+  try {
+    fn();
+    return GNSS_OK;
+  } catch (const Error& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return GNSS_ERR_DEVICE;
+  }
+}
+
+static bool good_len(int n) { return gnss::supported_len(n); }
+
+GNSS_API int gnss_create(const gnss_params* p, gnss_ctx** out) {
+  if (!p || !out) {
+    g_create_err = "null argument";
+    return GNSS_ERR_ARG;
+  }
+  *out = nullptr;
+  char msg[256];
+  if (!good_len(p->nx) || !good_len(p->ny) || !good_len(p->nz)) {
+    snprintf(msg, sizeof msg,
+             "grid %dx%dx%d: each size must be one of 16 24 32 48 64 96 128 192 256 384 512 1024", p->nx,
+             p->ny, p->nz);
+    g_create_err = msg;
+    return GNSS_ERR_ARG;
+  }
+  if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) { g_create_err = "box lengths must be > 0"; return GNSS_ERR_ARG; }
+  if (!(p->re > 0)) { g_create_err = "re must be > 0"; return GNSS_ERR_ARG; }
+  if (!(p->dt > 0)) { g_create_err = "dt must be > 0"; return GNSS_ERR_ARG; }
+  if (p->n_scalars < 0 || p->n_scalars > GNSS_MAX_SCALARS) {
+    g_create_err = "n_scalars must be in [0, 4]";
+    return GNSS_ERR_ARG;
+  }
+  for (int s = 0; s < p->n_scalars; ++s)
+    if (!(p->sc[s] > 0)) { g_create_err = "sc must be > 0"; return GNSS_ERR_ARG; }
+  if (p->precision != GNSS_FP64 && p->precision != GNSS_FP32) {
+    g_create_err = "precision must be GNSS_FP64 or GNSS_FP32";
+    return GNSS_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    g_create_err = "no HIP device available";
+    return GNSS_ERR_DEVICE;
+  }
+  gnss_ctx* c = new gnss_ctx();
+  c->p = *p;
+  try {
+    c->eng = p->precision == GNSS_FP64 ? gnss::make_engine_f64(*p) : gnss::make_engine_f32(*p);
+  } catch (const Error& e) {
+    g_create_err = e.what();
+    delete c;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_err = e.what();
+    delete c;
+    return GNSS_ERR_DEVICE;
+  }
+  *out = c;
+  return GNSS_OK;
+}
+
+GNSS_API int gnss_destroy(gnss_ctx* c) {
+  if (!c) return GNSS_OK;
+  delete c->eng;
+  delete c;
+  return GNSS_OK;
+}
+
+GNSS_API const char* gnss_last_error(const gnss_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+GNSS_API int gnss_get_sizes(const gnss_ctx* c, gnss_sizes* o) {
+  if (!c || !c->eng || !o) return GNSS_ERR_ARG;
+  c->eng->sizes(o);
+  return GNSS_OK;
+}
+
+GNSS_API int gnss_init_wavenumbers(gnss_ctx* c) { return guard(c, [&] { c->eng->init_wavenumbers(); }); }
+GNSS_API int gnss_get_wavenumbers(gnss_ctx* c, double* kx, double* ky, double* kz) {
+  return guard(c, [&] { c->eng->get_wavenumbers(kx, ky, kz); });
+}
+GNSS_API int gnss_get_mask(gnss_ctx* c, uint8_t* mask) {
+  if (!mask) return GNSS_ERR_ARG;
+  return guard(c, [&] { c->eng->get_mask(mask); });
+}
+GNSS_API int gnss_init_jet(gnss_ctx* c) { return guard(c, [&] { c->eng->init_jet(); }); }
+GNSS_API int gnss_forward(gnss_ctx* c, int f) { return guard(c, [&] { c->eng->forward(f); }); }
+GNSS_API int gnss_inverse(gnss_ctx* c, int f) { return guard(c, [&] { c->eng->inverse(f); }); }
+GNSS_API int gnss_dealias(gnss_ctx* c) { return guard(c, [&] { c->eng->dealias(); }); }
+GNSS_API int gnss_step(gnss_ctx* c, int n) { return guard(c, [&] { c->eng->step(n); }); }
+GNSS_API int gnss_stats(gnss_ctx* c, gnss_stats_out* o, double* yprof, double* spectrum) {
+  return guard(c, [&] { c->eng->stats(o, yprof, spectrum); });
+}
+GNSS_API int gnss_get_field(gnss_ctx* c, int f, int space, double* host) {
+  if (!host) return GNSS_ERR_ARG;
+  return guard(c, [&] { c->eng->get_field(f, space, host); });
+}
+GNSS_API int gnss_set_field(gnss_ctx* c, int f, int space, const double* host) {
+  if (!host) return GNSS_ERR_ARG;
+  return guard(c, [&] { c->eng->set_field(f, space, host); });
+}
+GNSS_API int gnss_get_time(const gnss_ctx* c, double* t, int64_t* step) {
+  if (!c || !c->eng) return GNSS_ERR_ARG;
+  c->eng->time(t, step);
+  return GNSS_OK;
+}
+GNSS_API int gnss_timing_enable(gnss_ctx* c, int on) { return guard(c, [&] { c->eng->timing_enable(on != 0); }); }
+GNSS_API int gnss_timing_report(gnss_ctx* c, char* buf, int len) {
+  if (!buf || len <= 0) return GNSS_ERR_ARG;
+  return guard(c, [&] {
+    std::string s = c->eng->timing_report();
+    if ((int)s.size() + 1 > len) throw Error(GNSS_ERR_ARG, "report buffer too small");
+    memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+GNSS_API int gnss_bench_fft(gnss_ctx* c, int reps, double* ms_fwd, double* ms_inv) {
+  return guard(c, [&] { c->eng->bench_fft(reps, ms_fwd, ms_inv); });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ensemble entry points are implemented in ensemble.hip.
+// ---------------------------------------------------------------------------------------------

@@ -1,0 +1,92 @@
+// engine.h — host-side engine of the pseudospectral step (one context = one HIP stream).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gnss.h"
+
+namespace gnss {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define GNSS_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::gnss::Error(GNSS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Per-kernel timing with HIP events on the engine's stream.
+class KernelTimer {
+ public:
+  void enable(bool on, hipStream_t s);
+  bool on() const { return on_; }
+  void begin(const char* name, double bytes);
+  void end();
+  std::string report();
+  ~KernelTimer();
+
+ private:
+  struct Rec {
+    std::string name;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  struct Agg {
+    long long launches = 0;
+    double ms = 0, bytes = 0;
+  };
+  void drain();
+  hipEvent_t get_event();
+  bool on_ = false;
+  hipStream_t stream_ = nullptr;
+  std::vector<Rec> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, Agg> agg_;
+};
+
+// Precision-independent interface used by the C ABI.
+class EngineBase {
+ public:
+  virtual ~EngineBase() {}
+  virtual void init_wavenumbers() = 0;
+  virtual void get_wavenumbers(double* kx, double* ky, double* kz) = 0;
+  virtual void get_mask(uint8_t* mask) = 0;
+  virtual void init_jet() = 0;
+  virtual void forward(int field) = 0;
+  virtual void inverse(int field) = 0;
+  virtual void dealias() = 0;
+  virtual void step(int nsteps) = 0;
+  virtual void stats(gnss_stats_out* out, double* yprof, double* spectrum) = 0;
+  virtual void get_field(int field, int space, double* host) = 0;
+  virtual void set_field(int field, int space, const double* host) = 0;
+  virtual void sizes(gnss_sizes* out) const = 0;
+  virtual void time(double* t, int64_t* step) const = 0;
+  virtual void timing_enable(bool on) = 0;
+  virtual std::string timing_report() = 0;
+  virtual void bench_fft(int reps, double* ms_fwd, double* ms_inv) = 0;
+  std::string err;
+};
+
+// Grid sizes the radix plans cover (products of 2s and 3s with a compiled plan).
+inline bool supported_len(int n) {
+  switch (n) {
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
+    case 384: case 512: case 1024:
+      return true;
+    default:
+      return false;
+  }
+}
+
+EngineBase* make_engine_f64(const gnss_params& p);
+EngineBase* make_engine_f32(const gnss_params& p);
+
+}  // namespace gnss

@@ -1,0 +1,6 @@
+// FP32 form of the engine: FFTs, products and RK updates in FP32, reductions in FP64.
+#include "engine_impl.h"
+
+namespace gnss {
+EngineBase* make_engine_f32(const gnss_params& p) { return new Engine<float>(p); }
+}  // namespace gnss

@@ -1,0 +1,6 @@
+// FP64 ("plain") form of the engine.
+#include "engine_impl.h"
+
+namespace gnss {
+EngineBase* make_engine_f64(const gnss_params& p) { return new Engine<double>(p); }
+}  // namespace gnss

@@ -1,0 +1,753 @@
+// engine_impl.h — Engine<T>: allocation, transforms, the RK3 step and statistics on one stream.
+//
+// Call structure kept from the reference driver (main.cu:84-252): create (allocate_memory +
+// plans) -> init_wavenumbers -> init_jet (IC + forward + dealias) -> step / stats -> destroy.
+#pragma once
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <sstream>
+
+#include "engine.h"
+#include "host_util.h"
+#include "kernels_fft.h"
+#include "kernels_step.h"
+
+namespace gnss {
+
+static const double RK_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+static const double RK_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+static const double RK_C[4] = {0.0, 1.0 / 3.0, 3.0 / 4.0, 1.0};
+
+inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+template <typename T>
+class Engine : public EngineBase {
+  using C = cx<T>;
+  static constexpr int LDS_MAX = 64 * 1024;
+
+ public:
+  explicit Engine(const gnss_params& p) : p_(p) {
+    nx_ = p.nx; ny_ = p.ny; nz_ = p.nz;
+    nzh_ = nz_ / 2 + 1;
+    S_ = p.n_scalars;
+    F_ = 3 + S_;
+    W_ = 6 + 3 * S_;
+    cx_ = col_chunk(nx_);
+    cy_ = col_chunk(ny_);
+    const int cmax = std::max(cx_, cy_);
+    kzs_ = round_up(nzh_, cmax);
+    g_.nx = nx_; g_.ny = ny_; g_.nz = nz_; g_.nzh = nzh_; g_.kzs = kzs_;
+    g_.keep_x = nx_ / 3; g_.keep_y = ny_ / 3; g_.keep_z = nz_ / 3;
+    int64_t ax, ay, az, g2;
+    mask_factors(nx_, ny_, nz_, ax, ay, az, g2);
+    g_.ax = ax; g_.ay = ay; g_.az = az; g_.g2 = g2;
+    kz_lim_ = std::min(round_up(g_.keep_z + 1, cmax), kzs_);
+    nsp_ = (long long)nx_ * ny_ * kzs_;
+    nph_ = (long long)nx_ * ny_ * nz_;
+    GNSS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    try {
+      allocate();
+    } catch (...) {
+      release();
+      throw;
+    }
+  }
+
+  ~Engine() override { release(); }
+
+  // ---------------------------------------------------------------------------------------
+  void sizes(gnss_sizes* o) const override {
+    o->n_fields = F_;
+    o->nzh = nzh_;
+    o->n_prof = 7 + 3 * S_;
+    o->n_bins = n_bins();
+    o->n_physical = nph_;
+    o->n_spectral = (long long)nx_ * ny_ * nzh_;
+  }
+  void time(double* t, int64_t* st) const override {
+    if (t) *t = t_;
+    if (st) *st = step_;
+  }
+
+  void init_wavenumbers() override {
+    const double pi = 3.14159265358979323846;
+    launch_wn(dkx_, nx_, nx_, 2.0 * pi / p_.lx);
+    launch_wn(dky_, ny_, ny_, 2.0 * pi / p_.ly);
+    launch_wn(dkz_, nz_, nzh_, 2.0 * pi / p_.lz);
+    cast_wn(dkx_, tkx_, nx_);
+    cast_wn(dky_, tky_, ny_);
+    cast_wn(dkz_, tkz_, nzh_);
+    sync();
+    have_k_ = true;
+  }
+
+  void get_wavenumbers(double* kx, double* ky, double* kz) override {
+    need_k();
+    if (kx) GNSS_HIP(hipMemcpy(kx, dkx_, sizeof(double) * nx_, hipMemcpyDeviceToHost));
+    if (ky) GNSS_HIP(hipMemcpy(ky, dky_, sizeof(double) * ny_, hipMemcpyDeviceToHost));
+    if (kz) GNSS_HIP(hipMemcpy(kz, dkz_, sizeof(double) * nzh_, hipMemcpyDeviceToHost));
+  }
+
+  void get_mask(uint8_t* mask) override {
+    need_k();
+    const long long n = (long long)nx_ * ny_ * nzh_;
+    unsigned char* d = (unsigned char*)scratch_bytes(n);
+    hipLaunchKernelGGL(k_mask_abi, dim3(cdiv(n, 256)), dim3(256), 0, s_, g_, d, n);
+    check_launch("k_mask_abi");
+    GNSS_HIP(hipMemcpyAsync(mask, d, n, hipMemcpyDeviceToHost, s_));
+    sync();
+  }
+
+  void init_jet() override {
+    need_k();
+    if (!(p_.theta > 0)) throw Error(GNSS_ERR_ARG, "theta must be > 0");
+    std::vector<HostIcEntry> h;
+    build_initial_condition(p_, h);
+    std::vector<IcEntry> e(h.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+      e[i].index = ((long long)h[i].ix * ny_ + h[i].iy) * kzs_ + h[i].iz;
+      e[i].field = h[i].field;
+      e[i].pad = 0;
+      e[i].re = h[i].re;
+      e[i].im = h[i].im;
+    }
+    for (int f = 0; f < F_; ++f) {
+      GNSS_HIP(hipMemsetAsync(q_[f], 0, sizeof(C) * nsp_, s_));
+      GNSS_HIP(hipMemsetAsync(dq_[f], 0, sizeof(C) * nsp_, s_));
+    }
+    if (!e.empty()) {
+      IcEntry* d = (IcEntry*)scratch_bytes(sizeof(IcEntry) * e.size());
+      GNSS_HIP(hipMemcpyAsync(d, e.data(), sizeof(IcEntry) * e.size(), hipMemcpyHostToDevice, s_));
+      hipLaunchKernelGGL(k_ic_scatter<T>, dim3(cdiv((long long)e.size(), 256)), dim3(256), 0, s_, d,
+                         (int)e.size(), ptrs(q_));
+      check_launch("k_ic_scatter");
+    }
+    sync();
+    t_ = 0;
+    step_ = 0;
+    have_ic_ = true;
+  }
+
+  void forward(int f) override {
+    check_field(f);
+    fwd3d(phys_[f], q_[f], false);
+    sync();
+  }
+
+  void inverse(int f) override {
+    check_field(f);
+    GNSS_HIP(hipMemcpyAsync(ws_[0], q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+    inv3d(ws_[0], phys_[f], false);
+    sync();
+  }
+
+  void dealias() override {
+    need_k();
+    hipLaunchKernelGGL(k_dealias<T>, dim3(cdiv(nsp_, 256)), dim3(256), 0, s_, g_, ptrs(q_), F_, nsp_);
+    check_launch("k_dealias");
+    sync();
+  }
+
+  void step(int nsteps) override {
+    need_k();
+    if (nsteps < 0) throw Error(GNSS_ERR_ARG, "nsteps must be >= 0");
+    if (nsteps == 0) return;
+    // the step starts from a dealiased state (deAlias at the top of solver_ps, solver.cu:704);
+    // every substage keeps it dealiased afterwards
+    timed("k_dealias", (double)nsp_ * F_ * sizeof(C), [&] {
+      hipLaunchKernelGGL(k_dealias<T>, dim3(cdiv(nsp_, 256)), dim3(256), 0, s_, g_, ptrs(q_), F_, nsp_);
+    });
+    for (int n = 0; n < nsteps; ++n) {
+      for (int s = 0; s < 3; ++s) substage(s);
+      step_ += 1;
+      t_ = (double)step_ * p_.dt;
+    }
+    // NaN/Inf watchdog
+    GNSS_HIP(hipMemsetAsync(d_flag_, 0, sizeof(int), s_));
+    hipLaunchKernelGGL(k_finite<T>, dim3(cdiv((long long)nx_ * ny_, 256)), dim3(256), 0, s_, g_,
+                       ptrs(q_), F_, d_flag_);
+    check_launch("k_finite");
+    int flag = 0;
+    GNSS_HIP(hipMemcpyAsync(&flag, d_flag_, sizeof(int), hipMemcpyDeviceToHost, s_));
+    sync();
+    if (flag) throw Error(GNSS_ERR_NUMERIC, "non-finite value in the state after step " + std::to_string(step_));
+  }
+
+  void stats(gnss_stats_out* o, double* yprof, double* spectrum) override {
+    need_k();
+    const double nn = (double)nx_ * ny_ * nz_;
+    const double inn2 = 1.0 / (nn * nn);
+    const int nb = n_bins();
+    // spectral sums per kx plane
+    timed("k_spec_sums", (double)F_ * nx_ * ny_ * nzh_ * sizeof(C), [&] {
+      hipLaunchKernelGGL(k_spec_sums<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, ptrs(q_), dkx_,
+                         dky_, dkz_, S_, inn2, d_part_);
+    });
+    ShellArgs sa;
+    sa.equal_box = (p_.lx == p_.ly && p_.ly == p_.lz) ? 1 : 0;
+    const double lm = std::max(p_.lx, std::max(p_.ly, p_.lz));
+    sa.fx = lm / p_.lx; sa.fy = lm / p_.ly; sa.fz = lm / p_.lz;
+    timed("k_spectrum", (double)F_ * nx_ * ny_ * nzh_ * sizeof(C), [&] {
+      hipLaunchKernelGGL(k_spectrum<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, ptrs(q_), S_, nb, sa,
+                         inn2, d_spec_);
+    });
+    // physical fields
+    for (int f = 0; f < F_; ++f) {
+      GNSS_HIP(hipMemcpyAsync(ws_[0], q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+      inv3d(ws_[0], phys_[f], false);
+    }
+    RPtrs<T> pp{};
+    for (int f = 0; f < F_; ++f) pp.p[f] = phys_[f];
+    timed("k_plane_stats", (double)F_ * nph_ * sizeof(T) * 2, [&] {
+      hipLaunchKernelGGL(k_plane_stats<T>, dim3(ny_), dim3(STAT_THREADS), 0, s_, g_, pp, S_,
+                         nx_ / p_.lx, ny_ / p_.ly, nz_ / p_.lz, d_prof_, d_cmax_);
+    });
+    std::vector<double> part((size_t)nx_ * SPEC_NQ), spec((size_t)nx_ * (1 + S_) * nb),
+        prof((size_t)(7 + 3 * S_) * ny_), cm((size_t)2 * ny_);
+    GNSS_HIP(hipMemcpyAsync(part.data(), d_part_, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s_));
+    GNSS_HIP(hipMemcpyAsync(spec.data(), d_spec_, sizeof(double) * spec.size(), hipMemcpyDeviceToHost, s_));
+    GNSS_HIP(hipMemcpyAsync(prof.data(), d_prof_, sizeof(double) * prof.size(), hipMemcpyDeviceToHost, s_));
+    GNSS_HIP(hipMemcpyAsync(cm.data(), d_cmax_, sizeof(double) * cm.size(), hipMemcpyDeviceToHost, s_));
+    sync();
+    double sum[SPEC_NQ] = {0};
+    for (int x = 0; x < nx_; ++x)
+      for (int qq = 0; qq < SPEC_NQ; ++qq) sum[qq] += part[(size_t)x * SPEC_NQ + qq];
+    const double nu = 1.0 / p_.re;
+    if (o) {
+      memset(o, 0, sizeof(*o));
+      o->t = t_;
+      o->step = step_;
+      o->ke = 0.5 * sum[0];
+      o->vrms = sqrt(sum[0] / 3.0);
+      o->epsilon = nu * sum[1];
+      o->enstrophy = 0.5 * sum[1];
+      o->lint = o->ke > 0 ? 3.0 * 3.14159265358979323846 / 4.0 * sum[2] / o->ke : 0.0;
+      o->lambda = o->epsilon > 0 ? sqrt(15.0 * nu * o->vrms * o->vrms / o->epsilon) : 0.0;
+      o->eta = o->epsilon > 0 ? sqrt(sqrt(nu * nu * nu / o->epsilon)) : 0.0;
+      o->divergence = sum[1] > 0 ? sqrt(sum[3] / sum[1]) : 0.0;
+      for (int s = 0; s < S_; ++s) {
+        o->chi[s] = 2.0 * (nu / p_.sc[s]) * sum[4 + 2 * s];
+        o->scalar_var[s] = sum[5 + 2 * s];
+      }
+      double cmx = 0, um = 0;
+      for (int y = 0; y < ny_; ++y) {
+        cmx = std::max(cmx, cm[2 * y]);
+        um = std::max(um, cm[2 * y + 1]);
+      }
+      o->cfl = p_.dt * cmx;
+      o->umax = um;
+    }
+    if (yprof) memcpy(yprof, prof.data(), sizeof(double) * prof.size());
+    if (spectrum) {
+      for (int s = 0; s <= S_; ++s)
+        for (int b = 0; b < nb; ++b) {
+          double acc = 0;
+          for (int x = 0; x < nx_; ++x) acc += spec[((size_t)x * (1 + S_) + s) * nb + b];
+          spectrum[(size_t)s * nb + b] = acc;
+        }
+    }
+  }
+
+  void get_field(int f, int space, double* host) override {
+    check_field(f);
+    if (space == GNSS_PHYSICAL) {
+      std::vector<T> h(nph_);
+      GNSS_HIP(hipMemcpyAsync(h.data(), phys_[f], sizeof(T) * nph_, hipMemcpyDeviceToHost, s_));
+      sync();
+      for (long long i = 0; i < nph_; ++i) host[i] = (double)h[i];
+      return;
+    }
+    if (space != GNSS_SPECTRAL) throw Error(GNSS_ERR_ARG, "bad space");
+    std::vector<C> h(nsp_);
+    GNSS_HIP(hipMemcpyAsync(h.data(), q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToHost, s_));
+    sync();
+    for (int y = 0; y < ny_; ++y)
+      for (int x = 0; x < nx_; ++x) {
+        const C* src = h.data() + ((size_t)x * ny_ + y) * kzs_;
+        double* dst = host + 2 * ((size_t)y * nx_ + x) * nzh_;
+        for (int k = 0; k < nzh_; ++k) {
+          dst[2 * k] = (double)src[k].x;
+          dst[2 * k + 1] = (double)src[k].y;
+        }
+      }
+  }
+
+  void set_field(int f, int space, const double* host) override {
+    check_field(f);
+    if (space == GNSS_PHYSICAL) {
+      std::vector<T> h(nph_);
+      for (long long i = 0; i < nph_; ++i) h[i] = (T)host[i];
+      GNSS_HIP(hipMemcpyAsync(phys_[f], h.data(), sizeof(T) * nph_, hipMemcpyHostToDevice, s_));
+      sync();
+      return;
+    }
+    if (space != GNSS_SPECTRAL) throw Error(GNSS_ERR_ARG, "bad space");
+    std::vector<C> h(nsp_);
+    memset((void*)h.data(), 0, sizeof(C) * nsp_);
+    for (int y = 0; y < ny_; ++y)
+      for (int x = 0; x < nx_; ++x) {
+        C* dst = h.data() + ((size_t)x * ny_ + y) * kzs_;
+        const double* src = host + 2 * ((size_t)y * nx_ + x) * nzh_;
+        for (int k = 0; k < nzh_; ++k) {
+          dst[k].x = (T)src[2 * k];
+          dst[k].y = (T)src[2 * k + 1];
+        }
+      }
+    GNSS_HIP(hipMemcpyAsync(q_[f], h.data(), sizeof(C) * nsp_, hipMemcpyHostToDevice, s_));
+    GNSS_HIP(hipMemsetAsync(dq_[f], 0, sizeof(C) * nsp_, s_));
+    sync();
+  }
+
+  void timing_enable(bool on) override {
+    sync();
+    timer_.enable(on, s_);
+  }
+  std::string timing_report() override {
+    sync();
+    return timer_.report();
+  }
+
+  void bench_fft(int reps, double* ms_fwd, double* ms_inv) override {
+    if (reps <= 0) throw Error(GNSS_ERR_ARG, "reps must be > 0");
+    hipEvent_t e0, e1, e2;
+    GNSS_HIP(hipEventCreate(&e0));
+    GNSS_HIP(hipEventCreate(&e1));
+    GNSS_HIP(hipEventCreate(&e2));
+    // warm-up pair
+    inv3d_keep(q_[0], ws_[0], phys_[0]);
+    fwd3d(phys_[0], ws_[1], false);
+    float tf = 0, ti = 0;
+    for (int r = 0; r < reps; ++r) {
+      GNSS_HIP(hipEventRecord(e0, s_));
+      GNSS_HIP(hipMemcpyAsync(ws_[0], q_[0], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+      GNSS_HIP(hipEventRecord(e1, s_));
+      inv3d(ws_[0], phys_[0], false);
+      GNSS_HIP(hipEventRecord(e2, s_));
+      GNSS_HIP(hipEventSynchronize(e2));
+      float a = 0, b = 0;
+      GNSS_HIP(hipEventElapsedTime(&a, e1, e2));
+      ti += a;
+      GNSS_HIP(hipEventRecord(e0, s_));
+      fwd3d(phys_[0], ws_[1], false);
+      GNSS_HIP(hipEventRecord(e1, s_));
+      GNSS_HIP(hipEventSynchronize(e1));
+      GNSS_HIP(hipEventElapsedTime(&b, e0, e1));
+      tf += b;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (ms_fwd) *ms_fwd = tf / reps;
+    if (ms_inv) *ms_inv = ti / reps;
+  }
+
+ private:
+  // -------------------------------------------------------------------------------------------
+  int col_chunk(int L) const {
+    int c = sizeof(T) == 8 ? 8 : 16;
+    while (c > 4 && (long long)L * c * (long long)sizeof(C) > LDS_MAX) c /= 2;
+    return c;
+  }
+  int n_bins() const {
+    if (p_.lx == p_.ly && p_.ly == p_.lz) {
+      long long m2 = (long long)(nx_ / 2) * (nx_ / 2) + (long long)(ny_ / 2) * (ny_ / 2) + (long long)(nz_ / 2) * (nz_ / 2);
+      long long r = (long long)sqrt((double)(4 * m2));
+      while (r * r > 4 * m2) --r;
+      while ((r + 1) * (r + 1) <= 4 * m2) ++r;
+      return (int)((r + 1) / 2) + 1;
+    }
+    const double lm = std::max(p_.lx, std::max(p_.ly, p_.lz));
+    const double a = (nx_ / 2) * lm / p_.lx, b = (ny_ / 2) * lm / p_.ly, c = (nz_ / 2) * lm / p_.lz;
+    return (int)floor(sqrt(a * a + b * b + c * c) + 0.5) + 1;
+  }
+
+  void need_k() const {
+    if (!have_k_) throw Error(GNSS_ERR_STATE, "call gnss_init_wavenumbers first");
+  }
+  void check_field(int f) const {
+    if (f < 0 || f >= F_) throw Error(GNSS_ERR_ARG, "field " + std::to_string(f) + " out of range");
+  }
+  void sync() { GNSS_HIP(hipStreamSynchronize(s_)); }
+  void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(GNSS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  }
+  template <typename Fn> void timed(const char* name, double bytes, Fn&& fn) {
+    if (timer_.on()) timer_.begin(name, bytes);
+    fn();
+    check_launch(name);
+    if (timer_.on()) timer_.end();
+  }
+
+  template <typename P> void* dalloc(P*& ptr, size_t bytes) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(GNSS_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    allocs_.push_back(v);
+    ptr = (P*)v;
+    return v;
+  }
+
+  void* scratch_bytes(size_t n) {
+    if (n > scratch_n_) {
+      if (scratch_) {
+        GNSS_HIP(hipStreamSynchronize(s_));
+        GNSS_HIP(hipFree(scratch_));
+        scratch_ = nullptr;
+      }
+      void* v = nullptr;
+      GNSS_HIP(hipMalloc(&v, n));
+      scratch_ = v;
+      scratch_n_ = n;
+    }
+    return scratch_;
+  }
+
+  void allocate() {
+    q_.assign(F_, nullptr); dq_.assign(F_, nullptr); nl_.assign(F_, nullptr);
+    phys_.assign(F_, nullptr); ws_.assign(W_, nullptr);
+    for (int f = 0; f < F_; ++f) {
+      dalloc(q_[f], sizeof(C) * nsp_);
+      dalloc(dq_[f], sizeof(C) * nsp_);
+      dalloc(nl_[f], sizeof(C) * nsp_);
+      dalloc(phys_[f], sizeof(T) * nph_);
+    }
+    for (int w = 0; w < W_; ++w) dalloc(ws_[w], sizeof(C) * nsp_);
+    for (void* v : allocs_) GNSS_HIP(hipMemsetAsync(v, 0, 1, s_));
+    for (int f = 0; f < F_; ++f) {
+      GNSS_HIP(hipMemsetAsync(q_[f], 0, sizeof(C) * nsp_, s_));
+      GNSS_HIP(hipMemsetAsync(dq_[f], 0, sizeof(C) * nsp_, s_));
+      GNSS_HIP(hipMemsetAsync(nl_[f], 0, sizeof(C) * nsp_, s_));
+      GNSS_HIP(hipMemsetAsync(phys_[f], 0, sizeof(T) * nph_, s_));
+    }
+    for (int w = 0; w < W_; ++w) GNSS_HIP(hipMemsetAsync(ws_[w], 0, sizeof(C) * nsp_, s_));
+    dalloc(dkx_, sizeof(double) * nx_);
+    dalloc(dky_, sizeof(double) * ny_);
+    dalloc(dkz_, sizeof(double) * nzh_);
+    dalloc(tkx_, sizeof(T) * nx_);
+    dalloc(tky_, sizeof(T) * ny_);
+    dalloc(tkz_, sizeof(T) * nzh_);
+    const int nb = n_bins();
+    dalloc(d_part_, sizeof(double) * nx_ * SPEC_NQ);
+    dalloc(d_spec_, sizeof(double) * (size_t)nx_ * (1 + S_) * nb);
+    dalloc(d_prof_, sizeof(double) * (size_t)(7 + 3 * S_) * ny_);
+    dalloc(d_cmax_, sizeof(double) * 2 * ny_);
+    dalloc(d_flag_, sizeof(int));
+    // twiddle tables for every length the passes use
+    for (int L : {nx_, ny_, nz_, nz_ / 2}) {
+      if (tw_.count(L)) continue;
+      std::vector<double> t;
+      twiddle_table(L, t);
+      std::vector<C> h(L);
+      for (int m = 0; m < L; ++m) {
+        h[m].x = (T)t[2 * m];
+        h[m].y = (T)t[2 * m + 1];
+      }
+      C* d = nullptr;
+      dalloc(d, sizeof(C) * L);
+      GNSS_HIP(hipMemcpyAsync(d, h.data(), sizeof(C) * L, hipMemcpyHostToDevice, s_));
+      tw_[L] = d;
+    }
+    sync();
+  }
+
+  void ensure_unfused_buffers() {
+    if (!wp_.empty()) return;
+    wp_.assign(W_, nullptr);
+    nlp_.assign(F_, nullptr);
+    for (int w = 0; w < W_; ++w) dalloc(wp_[w], sizeof(T) * nph_);
+    for (int f = 0; f < F_; ++f) dalloc(nlp_[f], sizeof(T) * nph_);
+  }
+
+  void release() {
+    if (s_) (void)hipStreamSynchronize(s_);
+    for (void* v : allocs_) (void)hipFree(v);
+    allocs_.clear();
+    if (scratch_) (void)hipFree(scratch_);
+    scratch_ = nullptr;
+    if (s_) (void)hipStreamDestroy(s_);
+    s_ = nullptr;
+  }
+
+  CPtrs<T> ptrs(const std::vector<C*>& v) const {
+    CPtrs<T> r{};
+    for (size_t i = 0; i < v.size() && i < (size_t)MAXW; ++i) r.p[i] = v[i];
+    return r;
+  }
+  RPtrs<T> rptrs(const std::vector<T*>& v) const {
+    RPtrs<T> r{};
+    for (size_t i = 0; i < v.size() && i < (size_t)MAXW; ++i) r.p[i] = v[i];
+    return r;
+  }
+
+  void launch_wn(double* d, int n, int len, double scale) {
+    hipLaunchKernelGGL(k_wavenumbers, dim3(cdiv(len, 256)), dim3(256), 0, s_, d, n, len, scale);
+    check_launch("k_wavenumbers");
+  }
+  void cast_wn(const double* in, T* out, int n) {
+    hipLaunchKernelGGL(k_cast_wavenumbers<T>, dim3(cdiv(n, 256)), dim3(256), 0, s_, in, out, n);
+    check_launch("k_cast_wavenumbers");
+  }
+
+  // -------------------------------------------------------------------------------------------
+  // Transform passes
+  // -------------------------------------------------------------------------------------------
+  struct ColSpec {
+    int axis;        // 0 = x, 1 = y
+    int sign;        // -1 forward, +1 inverse
+    int nchunks;     // column chunks processed
+    int outer_keep;  // kept-mode outer map (x pass: over ky), -1 = all
+    int in_keep, out_keep;
+  };
+
+  template <int L, int SIGN, int CC>
+  void col_launch(C* data, const ColArgs& a, int G) {
+    constexpr int TPC = L / Plan<L>::E;
+    const int threads = CC * TPC * G;
+    const size_t lds = (size_t)G * L * CC * sizeof(C);
+    const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
+    hipLaunchKernelGGL((k_col<T, L, SIGN, CC>), dim3((unsigned)blocks), dim3(threads), lds, s_, data,
+                       tw_.at(L), a);
+  }
+
+  template <int L, int SIGN>
+  void col_dispatch_c(C* data, const ColArgs& a, int CC, int G) {
+    if constexpr (sizeof(T) == 8) {
+      if (CC == 8) col_launch<L, SIGN, 8>(data, a, G);
+      else col_launch<L, SIGN, 4>(data, a, G);
+    } else {
+      if (CC == 16) col_launch<L, SIGN, 16>(data, a, G);
+      else col_launch<L, SIGN, 8>(data, a, G);
+    }
+  }
+
+  template <int SIGN>
+  void col_dispatch_l(int L, C* data, const ColArgs& a, int CC, int G) {
+    switch (L) {
+      case 16: col_dispatch_c<16, SIGN>(data, a, CC, G); break;
+      case 24: col_dispatch_c<24, SIGN>(data, a, CC, G); break;
+      case 32: col_dispatch_c<32, SIGN>(data, a, CC, G); break;
+      case 48: col_dispatch_c<48, SIGN>(data, a, CC, G); break;
+      case 64: col_dispatch_c<64, SIGN>(data, a, CC, G); break;
+      case 96: col_dispatch_c<96, SIGN>(data, a, CC, G); break;
+      case 128: col_dispatch_c<128, SIGN>(data, a, CC, G); break;
+      case 192: col_dispatch_c<192, SIGN>(data, a, CC, G); break;
+      case 256: col_dispatch_c<256, SIGN>(data, a, CC, G); break;
+      case 384: col_dispatch_c<384, SIGN>(data, a, CC, G); break;
+      case 512: col_dispatch_c<512, SIGN>(data, a, CC, G); break;
+      case 1024: col_dispatch_c<1024, SIGN>(data, a, CC, G); break;
+      default: throw Error(GNSS_ERR_UNSUPPORTED, "unsupported transform length " + std::to_string(L));
+    }
+  }
+
+  static int elems_per_thread(int L) {
+    switch (L) {
+      case 8: return Plan<8>::E;
+      case 12: return Plan<12>::E;
+      case 16: return Plan<16>::E;
+      case 24: return Plan<24>::E;
+      case 32: return Plan<32>::E;
+      case 48: return Plan<48>::E;
+      case 64: return Plan<64>::E;
+      case 96: return Plan<96>::E;
+      case 128: return Plan<128>::E;
+      case 192: return Plan<192>::E;
+      case 256: return Plan<256>::E;
+      case 384: return Plan<384>::E;
+      case 512: return Plan<512>::E;
+      case 1024: return Plan<1024>::E;
+      default: throw Error(GNSS_ERR_UNSUPPORTED, "unsupported transform length " + std::to_string(L));
+    }
+  }
+
+  void col_pass(C* data, const ColSpec& sp) {
+    const int L = sp.axis == 0 ? nx_ : ny_;
+    const int CC = sp.axis == 0 ? cx_ : cy_;
+    const int TPC = L / elems_per_thread(L);
+    ColArgs a;
+    if (sp.axis == 1) {
+      a.estride = kzs_;
+      a.ostride = (long long)ny_ * kzs_;
+      a.nouter = nx_;
+      a.outer_keep = -1;
+      a.outer_n = nx_;
+    } else {
+      a.estride = (long long)ny_ * kzs_;
+      a.ostride = kzs_;
+      a.outer_keep = sp.outer_keep;
+      a.outer_n = ny_;
+      a.nouter = sp.outer_keep < 0 ? ny_ : 2 * sp.outer_keep + 1;
+    }
+    a.nchunks = sp.nchunks;
+    a.in_keep = sp.in_keep;
+    a.out_keep = sp.out_keep;
+    int G = std::max(1, 256 / (CC * TPC));
+    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
+    a.G = G;
+    // algorithmic bytes: rows loaded + rows stored
+    auto rows = [&](int keep) { return keep < 0 ? L : std::min(L, 2 * keep + 1); };
+    const double bytes = (double)a.nouter * a.nchunks * CC * (rows(sp.in_keep) + rows(sp.out_keep)) * sizeof(C);
+    const char* name = sp.axis == 0 ? (sp.sign < 0 ? "fft_x_fwd" : "fft_x_inv") : (sp.sign < 0 ? "fft_y_fwd" : "fft_y_inv");
+    timed(name, bytes, [&] {
+      if (sp.sign < 0) col_dispatch_l<-1>(L, data, a, CC, G);
+      else col_dispatch_l<+1>(L, data, a, CC, G);
+    });
+  }
+
+  template <int NZ>
+  void zr2c_launch(const T* in, C* out, int out_lim) {
+    constexpr int M = NZ / 2;
+    constexpr int TPC = M / Plan<M>::E;
+    int G = std::max(1, 256 / TPC);
+    while (G > 1 && (long long)G * zrow_stride(M) * (long long)sizeof(C) > LDS_MAX) --G;
+    const long long rows = (long long)nx_ * ny_;
+    hipLaunchKernelGGL((k_r2c_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G),
+                       (size_t)G * zrow_stride(M) * sizeof(C), s_, in, out, tw_.at(M), tw_.at(NZ), rows,
+                       kzs_, out_lim, G);
+  }
+  template <int NZ>
+  void zc2r_launch(const C* in, T* out, int in_lim, T scale) {
+    constexpr int M = NZ / 2;
+    constexpr int TPC = M / Plan<M>::E;
+    int G = std::max(1, 256 / TPC);
+    while (G > 1 && (long long)G * zrow_stride(M) * (long long)sizeof(C) > LDS_MAX) --G;
+    const long long rows = (long long)nx_ * ny_;
+    hipLaunchKernelGGL((k_c2r_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G),
+                       (size_t)G * zrow_stride(M) * sizeof(C), s_, in, out, tw_.at(M), tw_.at(NZ), rows,
+                       kzs_, in_lim, scale, G);
+  }
+
+#define GNSS_Z_SWITCH(FN, ...)                                                                 \
+  switch (nz_) {                                                                               \
+    case 16: FN<16>(__VA_ARGS__); break;                                                       \
+    case 24: FN<24>(__VA_ARGS__); break;                                                       \
+    case 32: FN<32>(__VA_ARGS__); break;                                                       \
+    case 48: FN<48>(__VA_ARGS__); break;                                                       \
+    case 64: FN<64>(__VA_ARGS__); break;                                                       \
+    case 96: FN<96>(__VA_ARGS__); break;                                                       \
+    case 128: FN<128>(__VA_ARGS__); break;                                                     \
+    case 192: FN<192>(__VA_ARGS__); break;                                                     \
+    case 256: FN<256>(__VA_ARGS__); break;                                                     \
+    case 384: FN<384>(__VA_ARGS__); break;                                                     \
+    case 512: FN<512>(__VA_ARGS__); break;                                                     \
+    case 1024: FN<1024>(__VA_ARGS__); break;                                                   \
+    default: throw Error(GNSS_ERR_UNSUPPORTED, "unsupported nz " + std::to_string(nz_));      \
+  }
+
+  void z_r2c(const T* in, C* out, int out_lim) {
+    const double bytes = (double)nx_ * ny_ * ((double)nz_ * sizeof(T) + (double)std::min(out_lim, kzs_) * sizeof(C));
+    timed("fft_z_r2c", bytes, [&] { GNSS_Z_SWITCH(zr2c_launch, in, out, out_lim); });
+  }
+  void z_c2r(const C* in, T* out, int in_lim, T scale) {
+    const double bytes = (double)nx_ * ny_ * ((double)nz_ * sizeof(T) + (double)std::min(in_lim, nzh_) * sizeof(C));
+    timed("fft_z_c2r", bytes, [&] { GNSS_Z_SWITCH(zc2r_launch, in, out, in_lim, scale); });
+  }
+
+  // Forward 3D R2C: in (physical) -> out (spectral).  prune: only the kept box is produced.
+  void fwd3d(const T* in, C* out, bool prune) {
+    if (!prune) {
+      z_r2c(in, out, kzs_);
+      col_pass(out, ColSpec{1, -1, kzs_ / cy_, -1, -1, -1});
+      col_pass(out, ColSpec{0, -1, kzs_ / cx_, -1, -1, -1});
+      return;
+    }
+    z_r2c(in, out, kz_lim_);
+    col_pass(out, ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
+    col_pass(out, ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
+  }
+
+  // Inverse 3D C2R with 1/(nx ny nz): spec is overwritten (used as the work buffer).
+  void inv3d(C* spec, T* out, bool prune) {
+    const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
+    if (!prune) {
+      col_pass(spec, ColSpec{0, +1, kzs_ / cx_, -1, -1, -1});
+      col_pass(spec, ColSpec{1, +1, kzs_ / cy_, -1, -1, -1});
+      z_c2r(spec, out, nzh_, scale);
+      return;
+    }
+    col_pass(spec, ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+    col_pass(spec, ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+    z_c2r(spec, out, kz_lim_, scale);
+  }
+
+  void inv3d_keep(const C* spec, C* work, T* out) {
+    GNSS_HIP(hipMemcpyAsync(work, spec, sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+    inv3d(work, out, false);
+  }
+
+  // -------------------------------------------------------------------------------------------
+  // One RK3 substage: nonlinear terms of the current state, then the IF-RK update.
+  // -------------------------------------------------------------------------------------------
+  Box kept_box() const {
+    Box b;
+    b.bx = 2 * g_.keep_x + 1;
+    b.by = 2 * g_.keep_y + 1;
+    b.bz = kz_lim_;
+    b.kx_keep = g_.keep_x;
+    b.ky_keep = g_.keep_y;
+    return b;
+  }
+
+  void substage(int s) {
+    ensure_unfused_buffers();
+    const Box b = kept_box();
+    const long long nb = (long long)b.bx * b.by * b.bz;
+    timed("k_prologue", (double)nb * (F_ + W_) * sizeof(C), [&] {
+      hipLaunchKernelGGL(k_prologue<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_),
+                         ptrs(ws_), tkx_, tky_, tkz_, S_, nb);
+    });
+    for (int w = 0; w < W_; ++w) inv3d(ws_[w], wp_[w], true);
+    timed("k_products", (double)nph_ * (W_ + F_) * sizeof(T), [&] {
+      hipLaunchKernelGGL(k_products<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, rptrs(wp_),
+                         rptrs(nlp_), S_, nph_);
+    });
+    for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
+    RkCoef c;
+    c.a = RK_A[s];
+    c.b = RK_B[s];
+    c.tau = (RK_C[s + 1] - RK_C[s]) * p_.dt;
+    c.dt = p_.dt;
+    c.first = s == 0;
+    c.last = s == 2;
+    const double nu = 1.0 / p_.re;
+    for (int f = 0; f < MAXF; ++f) c.nu[f] = 0;
+    for (int f = 0; f < 3; ++f) c.nu[f] = nu;
+    for (int q = 0; q < S_; ++q) c.nu[3 + q] = nu / p_.sc[q];
+    const double rk_bytes = (double)nb * F_ * sizeof(C) * (c.first ? 3 : (c.last ? 4 : 5));
+    timed("k_rk", rk_bytes, [&] {
+      hipLaunchKernelGGL(k_rk<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_), ptrs(dq_),
+                         ptrs(nl_), tkx_, tky_, tkz_, c, F_, nb);
+    });
+  }
+
+  // -------------------------------------------------------------------------------------------
+  gnss_params p_;
+  int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8;
+  int kz_lim_ = 0;
+  long long nsp_ = 0, nph_ = 0;
+  Geom g_{};
+  hipStream_t s_ = nullptr;
+  std::vector<C*> q_, dq_, ws_, nl_;
+  std::vector<T*> wp_, nlp_, phys_;
+  std::map<int, C*> tw_;
+  double *dkx_ = nullptr, *dky_ = nullptr, *dkz_ = nullptr;
+  T *tkx_ = nullptr, *tky_ = nullptr, *tkz_ = nullptr;
+  double *d_part_ = nullptr, *d_spec_ = nullptr, *d_prof_ = nullptr, *d_cmax_ = nullptr;
+  int* d_flag_ = nullptr;
+  void* scratch_ = nullptr;
+  size_t scratch_n_ = 0;
+  std::vector<void*> allocs_;
+  bool have_k_ = false, have_ic_ = false;
+  double t_ = 0;
+  int64_t step_ = 0;
+  KernelTimer timer_;
+};
+
+}  // namespace gnss

@@ -1,0 +1,237 @@
+// fft_device.h — device building blocks of the VALU radix FFT passes (gfx950).
+//
+// Replaces the cuFFT plans of the reference (fftfuncs.cu:133-231) with hand-written Stockham
+// autosort transforms.  A transform of length L is computed by TPC = L/E threads; thread t holds
+// E complex values in "strided" order: register e holds position t + e*TPC.  Each radix stage
+// works entirely in registers; between stages the values are exchanged through LDS.  The input
+// of the first stage and the output of the last stage are both in strided order, so a pass loads
+// from and stores to HBM directly from registers: the lanes of a wavefront cover consecutive
+// columns (strided passes) or consecutive positions (contiguous passes), which keeps every HBM
+// access in runs of at least 128 bytes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gnss {
+
+template <typename T> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+template <typename T> using cx = typename cplx_t<T>::type;
+
+template <typename T> __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { a.x += b.x; a.y += b.y; return a; }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { a.x -= b.x; a.y -= b.y; return a; }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { a.y = -a.y; return a; }
+template <typename V, typename S> __device__ __forceinline__ V cscale(V a, S s) { a.x *= s; a.y *= s; return a; }
+// multiply by (SIGN * i)
+template <int SIGN, typename V> __device__ __forceinline__ V cmul_si(V a) {
+  V r;
+  if (SIGN < 0) { r.x = a.y; r.y = -a.x; } else { r.x = -a.y; r.y = a.x; }
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Small DFTs, out[q] = sum_r a[r] exp(SIGN 2 pi i r q / R), in place.
+// ---------------------------------------------------------------------------------------------
+template <int R, int SIGN, typename V> struct Dft;
+
+template <int SIGN, typename V> struct Dft<2, SIGN, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    V t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <int SIGN, typename V> struct Dft<3, SIGN, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = decltype(a[0].x);
+    const T s = (T)0.86602540378443864676372317075294;
+    V t = cadd(a[1], a[2]);
+    V d = csub(a[1], a[2]);
+    V m = csub(a[0], cscale(t, (T)0.5));
+    V jd = cscale(cmul_si<SIGN>(d), s);
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, jd);
+    a[2] = csub(m, jd);
+  }
+};
+
+template <int SIGN, typename V> struct Dft<4, SIGN, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    V t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    V t2 = cadd(a[1], a[3]), t3 = cmul_si<SIGN>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <int SIGN, typename V> struct Dft<8, SIGN, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = decltype(a[0].x);
+    const T h = (T)0.70710678118654752440084436210485;
+    V e[4] = {a[0], a[2], a[4], a[6]};
+    V o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, SIGN, V>::run(e);
+    Dft<4, SIGN, V>::run(o);
+    // o[q] *= w8^q, w8 = exp(SIGN 2 pi i / 8)
+    V o1 = o[1];
+    o[1].x = h * (o1.x - (T)SIGN * o1.y);
+    o[1].y = h * (o1.y + (T)SIGN * o1.x);
+    o[2] = cmul_si<SIGN>(o[2]);
+    V o3 = o[3];
+    o[3].x = h * (-o3.x - (T)SIGN * o3.y);
+    o[3].y = h * (-o3.y + (T)SIGN * o3.x);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = cadd(e[q], o[q]);
+      a[q + 4] = csub(e[q], o[q]);
+    }
+  }
+};
+
+template <int SIGN, typename V> struct Dft<16, SIGN, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = decltype(a[0].x);
+    const T c1 = (T)0.92387953251128675612818318939679;  // cos(pi/8)
+    const T s1 = (T)0.38268343236508977172845998403040;  // sin(pi/8)
+    const T h = (T)0.70710678118654752440084436210485;
+    // 16 = 4 x 4: columns r (input index 4m + r), DFT4 over m, twiddle w16^(r q), DFT4 over r
+    V b[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) b[r][m] = a[4 * m + r];
+      Dft<4, SIGN, V>::run(b[r]);
+    }
+    // twiddles w16^(r q) for r, q in 1..3; w16^j = cos(2 pi j/16) + SIGN i sin(2 pi j/16)
+    const T cw[10] = {(T)1, c1, h, s1, (T)0, -s1, -h, -c1, (T)-1, -c1};
+    const T sw[10] = {(T)0, s1, h, c1, (T)1, c1, h, s1, (T)0, -s1};
+#pragma unroll
+    for (int r = 1; r < 4; ++r)
+#pragma unroll
+      for (int q = 1; q < 4; ++q) {
+        const int j = r * q;  // <= 9
+        V w;
+        w.x = cw[j];
+        w.y = (T)SIGN * sw[j];
+        b[r][q] = cmul(b[r][q], w);
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      V c[4] = {b[0][q], b[1][q], b[2][q], b[3][q]};
+      Dft<4, SIGN, V>::run(c);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) a[q + 4 * p] = c[p];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Compile-time plans: radix sequence and elements per thread for each supported length.
+// ---------------------------------------------------------------------------------------------
+template <int... Rs> struct Radices {};
+
+template <int L> struct Plan;
+template <> struct Plan<8> { static constexpr int E = 8; using R = Radices<8>; };
+template <> struct Plan<12> { static constexpr int E = 12; using R = Radices<4, 3>; };
+template <> struct Plan<16> { static constexpr int E = 4; using R = Radices<4, 4>; };
+template <> struct Plan<24> { static constexpr int E = 12; using R = Radices<4, 2, 3>; };
+template <> struct Plan<32> { static constexpr int E = 8; using R = Radices<8, 4>; };
+template <> struct Plan<48> { static constexpr int E = 12; using R = Radices<4, 4, 3>; };
+template <> struct Plan<64> { static constexpr int E = 8; using R = Radices<8, 8>; };
+template <> struct Plan<96> { static constexpr int E = 12; using R = Radices<4, 4, 2, 3>; };
+template <> struct Plan<128> { static constexpr int E = 8; using R = Radices<8, 4, 4>; };
+template <> struct Plan<192> { static constexpr int E = 12; using R = Radices<4, 4, 4, 3>; };
+template <> struct Plan<256> { static constexpr int E = 8; using R = Radices<8, 8, 4>; };
+template <> struct Plan<384> { static constexpr int E = 12; using R = Radices<4, 4, 4, 2, 3>; };
+template <> struct Plan<512> { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
+template <> struct Plan<1024> { static constexpr int E = 16; using R = Radices<16, 8, 8>; };
+
+// Twiddle read: table tw[m] = exp(-2 pi i m / L); the inverse uses the conjugate.
+template <int SIGN, typename V> __device__ __forceinline__ V twid(const V* __restrict__ tw, int m) {
+  V w = tw[m];
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+// LDS index maps: column tile [pos][C] or row tile [row][pos] with padding.
+struct LdsCol {
+  int base, C;
+  __device__ __forceinline__ int operator()(int pos) const { return base + pos * C; }
+};
+struct LdsRow {
+  int base;
+  __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+};
+
+// One Stockham stage with radix R after a span NS; recursion over the remaining radices.
+template <typename T, int L, int E, int SIGN, int NS, typename Idx, int R, int... Rest>
+struct Stages {
+  __device__ __forceinline__ static void run(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
+                                             const cx<T>* __restrict__ tw) {
+    constexpr int TPC = L / E;
+    constexpr int NB = E / R;
+    constexpr bool LAST = sizeof...(Rest) == 0;
+    static_assert(E % R == 0, "radix must divide the elements per thread");
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * TPC;
+      const int k = (NS == 1) ? 0 : (j % NS);
+      cx<T> a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = cmul(a[r], twid<SIGN>(tw, r * k * (L / (NS * R))));
+      }
+      Dft<R, SIGN, cx<T>>::run(a);
+      if (LAST) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[b + q * NB] = a[q];
+      } else {
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[idx(base + q * NS)] = a[q];
+      }
+    }
+    if constexpr (!LAST) {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + e * TPC)];
+      __syncthreads();
+      Stages<T, L, E, SIGN, NS * R, Idx, Rest...>::run(v, lds, t, idx, tw);
+    }
+  }
+};
+
+template <typename T, int L, int SIGN, typename Idx, typename Rad> struct FftL;
+template <typename T, int L, int SIGN, typename Idx, int... Rs>
+struct FftL<T, L, SIGN, Idx, Radices<Rs...>> {
+  __device__ __forceinline__ static void run(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
+                                             const cx<T>* __restrict__ tw) {
+    Stages<T, L, Plan<L>::E, SIGN, 1, Idx, Rs...>::run(v, lds, t, idx, tw);
+  }
+};
+
+// Transform of length L held in strided order by TPC = L / Plan<L>::E threads.
+// Contains __syncthreads(): every thread of the block must call it.
+template <typename T, int L, int SIGN, typename Idx>
+__device__ __forceinline__ void fft(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
+                                    const cx<T>* __restrict__ tw) {
+  FftL<T, L, SIGN, Idx, typename Plan<L>::R>::run(v, lds, t, idx, tw);
+}
+
+// Signed mode index of storage index i along an axis of length n.
+__device__ __forceinline__ int mode_of(int i, int n) { return i < (n >> 1) ? i : i - n; }
+
+}  // namespace gnss

@@ -1,0 +1,419 @@
+// kernels_step.h — pointwise step kernels and FP64 statistics reductions.
+#pragma once
+#include "fft_device.h"
+
+namespace gnss {
+
+constexpr int MAXF = 3 + 4;           // velocity + up to 4 scalars
+constexpr int MAXW = 6 + 3 * 4;       // u, v, w, omega, grad phi per scalar
+
+template <typename T> struct CPtrs { cx<T>* p[MAXW]; };
+template <typename T> struct RPtrs { T* p[MAXW]; };
+
+// Grid geometry and the kept box of the 2/3 rule.
+struct Geom {
+  int nx, ny, nz, nzh, kzs;
+  int keep_x, keep_y, keep_z;   // floor(n_i / 3): the largest |m_i| the mask can keep
+  long long ax, ay, az, g2;     // mask: keep iff 9 * ((mx ax)^2 + (my ay)^2 + (mz az)^2) <= g2
+};
+
+// The dealiasing rule in integers: 9 * sum_i (m_i G / n_i)^2 <= G^2, G = lcm(nx, ny, nz).
+// Follows deAliasKernel_mgpu (solver.cu:17-42) with the 2/3 filter (dnsparams.h:65).
+__device__ __forceinline__ bool kept(const Geom& g, int mx, int my, int mz) {
+  const long long a = mx * g.ax, b = my * g.ay, c = mz * g.az;
+  return 9 * (a * a + b * b + c * c) <= g.g2;
+}
+
+// A box of spectral modes: outer kx (kept-mode map when kx_keep >= 0), ky likewise, kz < nkz.
+struct Box {
+  int bx, by, bz;       // extents
+  int kx_keep, ky_keep; // -1 = all
+};
+__device__ __forceinline__ void box_index(const Geom& g, const Box& b, long long i, int& ix,
+                                          int& iy, int& iz) {
+  iz = (int)(i % b.bz);
+  long long r = i / b.bz;
+  const int oy = (int)(r % b.by);
+  const int ox = (int)(r / b.by);
+  ix = b.kx_keep < 0 ? ox : (ox <= b.kx_keep ? ox : g.nx - (2 * b.kx_keep + 1) + ox);
+  iy = b.ky_keep < 0 ? oy : (oy <= b.ky_keep ? oy : g.ny - (2 * b.ky_keep + 1) + oy);
+}
+
+// i * k * f for real k: (-k f.y, k f.x)
+template <typename V, typename T> __device__ __forceinline__ V ik(T k, V f) {
+  V r;
+  r.x = -k * f.y;
+  r.y = k * f.x;
+  return r;
+}
+
+// Spectral prologue of a substage: u, v, w, omega = i k x u^ (calcOmega*Kernel_mgpu,
+// solver.cu:68-114) and grad phi = i k phi^ (gradientKernel_mgpu, solver.cu:233-256).
+template <typename T>
+__global__ void k_prologue(Geom g, Box b, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv,
+                           const T* __restrict__ kyv, const T* __restrict__ kzv, int S,
+                           long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ix, iy, iz;
+  box_index(g, b, i, ix, iy, iz);
+  const long long o = ((long long)ix * g.ny + iy) * g.kzs + iz;
+  const T kx = kxv[ix], ky = kyv[iy], kz = kzv[iz];
+  const cx<T> u = q.p[0][o], v = q.p[1][o], w = q.p[2][o];
+  ws.p[0][o] = u;
+  ws.p[1][o] = v;
+  ws.p[2][o] = w;
+  ws.p[3][o] = csub(ik(ky, w), ik(kz, v));
+  ws.p[4][o] = csub(ik(kz, u), ik(kx, w));
+  ws.p[5][o] = csub(ik(kx, v), ik(ky, u));
+  for (int s = 0; s < S; ++s) {
+    const cx<T> ph = q.p[3 + s][o];
+    ws.p[6 + 3 * s][o] = ik(kx, ph);
+    ws.p[7 + 3 * s][o] = ik(ky, ph);
+    ws.p[8 + 3 * s][o] = ik(kz, ph);
+  }
+}
+
+// Physical products: u x w (crossProductKernel_mgpu, solver.cu:139-164, with the sign of
+// computeRHSKernel_mgpu folded in) and u . grad phi (dotKernel_mgpu, solver.cu:342-356).
+template <typename T>
+__global__ void k_products(RPtrs<T> p, RPtrs<T> nl, int S, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T u = p.p[0][i], v = p.p[1][i], w = p.p[2][i];
+  const T ox = p.p[3][i], oy = p.p[4][i], oz = p.p[5][i];
+  nl.p[0][i] = v * oz - w * oy;
+  nl.p[1][i] = w * ox - u * oz;
+  nl.p[2][i] = u * oy - v * ox;
+  for (int s = 0; s < S; ++s) nl.p[3 + s][i] = u * p.p[6 + 3 * s][i] + v * p.p[7 + 3 * s][i] + w * p.p[8 + 3 * s][i];
+}
+
+template <typename T> __device__ __forceinline__ T dexp(T x);
+template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+
+struct RkCoef {
+  double a, b, tau, dt;   // dq <- a dq + dt rhs;  q <- E (q + b dq);  dq <- E dq;  E = exp(-nu k^2 tau)
+  int first, last;        // first: a = 0 (dq not read); last: dq not written
+  double nu[MAXF];        // viscosity (velocity) and diffusivities (scalars)
+};
+
+// Projection + scalar sign (computeRHSKernel_mgpu, solver.cu:461-521), mask (solver.cu:17-42)
+// and one low-storage RK3 substage with exact integrating factors (replaces solver.cu:547-657).
+template <typename T>
+__global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kxv,
+                     const T* __restrict__ kyv, const T* __restrict__ kzv, RkCoef c, int F,
+                     long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ix, iy, iz;
+  box_index(g, b, i, ix, iy, iz);
+  const long long o = ((long long)ix * g.ny + iy) * g.kzs + iz;
+  const int mz = mode_of(iz, g.nz);
+  if (iz >= g.nzh || !kept(g, mode_of(ix, g.nx), mode_of(iy, g.ny), mz)) {
+    for (int f = 0; f < F; ++f) {
+      q.p[f][o] = mk<T>(0, 0);
+      if (!c.last) dq.p[f][o] = mk<T>(0, 0);
+    }
+    return;
+  }
+  const T kx = kxv[ix], ky = kyv[iy], kz = kzv[iz];
+  const T k2 = kx * kx + ky * ky + kz * kz;
+  cx<T> rhs[MAXF];
+  if (k2 == (T)0) {
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f) rhs[f] = mk<T>(0, 0);
+  } else {
+    const cx<T> n0 = nl.p[0][o], n1 = nl.p[1][o], n2 = nl.p[2][o];
+    cx<T> kn;
+    kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
+    kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
+    rhs[0] = csub(n0, cscale(kn, kx));
+    rhs[1] = csub(n1, cscale(kn, ky));
+    rhs[2] = csub(n2, cscale(kn, kz));
+#pragma unroll
+    for (int f = 3; f < MAXF; ++f) {
+      const cx<T> v = f < F ? nl.p[f][o] : mk<T>(0, 0);
+      rhs[f] = mk<T>(-v.x, -v.y);
+    }
+  }
+  const T a = (T)c.a, bb = (T)c.b, dt = (T)c.dt;
+#pragma unroll
+  for (int f = 0; f < MAXF; ++f) {
+    if (f >= F) break;
+    const T e = dexp<T>(-(T)c.nu[f] * k2 * (T)c.tau);
+    cx<T> d = cscale(rhs[f], dt);
+    if (!c.first) {
+      const cx<T> old = dq.p[f][o];
+      d.x = a * old.x + d.x;
+      d.y = a * old.y + d.y;
+    }
+    cx<T> qq = q.p[f][o];
+    qq.x = e * (qq.x + bb * d.x);
+    qq.y = e * (qq.y + bb * d.y);
+    q.p[f][o] = qq;
+    if (!c.last) dq.p[f][o] = cscale(d, e);
+  }
+}
+
+// Zero every mode the 2/3 rule removes (and the kz padding): deAlias (solver.cu:44-66).
+template <typename T>
+__global__ void k_dealias(Geom g, CPtrs<T> q, int F, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int iz = (int)(i % g.kzs);
+  const long long r = i / g.kzs;
+  const int iy = (int)(r % g.ny), ix = (int)(r / g.ny);
+  if (iz < g.nzh && kept(g, mode_of(ix, g.nx), mode_of(iy, g.ny), mode_of(iz, g.nz))) return;
+  for (int f = 0; f < F; ++f) q.p[f][i] = mk<T>(0, 0);
+}
+
+// Mask in the ABI order [ky][kx][kz < nzh].
+static __global__ void k_mask_abi(Geom g, unsigned char* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int iz = (int)(i % g.nzh);
+  const long long r = i / g.nzh;
+  const int ix = (int)(r % g.nx), iy = (int)(r / g.nx);
+  out[i] = kept(g, mode_of(ix, g.nx), mode_of(iy, g.ny), mode_of(iz, g.nz)) ? 1 : 0;
+}
+
+// k_i = (2 pi / L_i) * m_i in FP64 (waveNumber_kernel, initialize.cu:678-691); `scale` is the
+// host-computed 2 pi / L_i so the product is the same IEEE operation as on the host.
+static __global__ void k_wavenumbers(double* __restrict__ k, int n, int len, double scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) k[i] = scale * (double)mode_of(i, n);
+}
+
+template <typename T>
+__global__ void k_cast_wavenumbers(const double* __restrict__ in, T* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)in[i];
+}
+
+struct IcEntry {
+  long long index;  // device linear index [kx][ky][kz]
+  int field;
+  int pad;
+  double re, im;
+};
+
+template <typename T>
+__global__ void k_ic_scatter(const IcEntry* __restrict__ e, int n, CPtrs<T> q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const IcEntry x = e[i];
+  q.p[x.field][x.index] = mk<T>((T)x.re, (T)x.im);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Statistics: deterministic FP64 reductions (fixed per-thread order + fixed tree), one block per
+// kx plane; the host sums the per-plane partials in plane order.
+// ---------------------------------------------------------------------------------------------
+constexpr int STAT_THREADS = 256;
+
+// Fixed-order tree sum of NQ per-thread values over the block; every thread receives all NQ
+// totals in `out` (registers).  Reuses `sh`, so it starts with a barrier.
+template <int NQ>
+__device__ __forceinline__ void block_sum(const double (&acc)[NQ], double* sh, double (&out)[NQ]) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  for (int qq = 0; qq < NQ; ++qq) sh[qq * STAT_THREADS + tid] = acc[qq];
+  __syncthreads();
+  for (int s = STAT_THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int qq = 0; qq < NQ; ++qq) sh[qq * STAT_THREADS + tid] += sh[qq * STAT_THREADS + tid + s];
+    __syncthreads();
+  }
+  for (int qq = 0; qq < NQ; ++qq) out[qq] = sh[qq * STAT_THREADS];
+}
+
+constexpr int SPEC_NQ = 4 + 2 * 4;
+
+// Sums per kx plane (calcVrms/Epsilon/IntegralLength/ScalarDissipation kernels,
+// statistics.cu:1494-1845): [0] sum w|u|^2, [1] sum w k^2|u|^2, [2] sum w|u|^2/(2|k|),
+// [3] sum w|k.u|^2, [4+2s] sum w k^2|phi|^2, [5+2s] sum_{k!=0} w|phi|^2 — all divided by NN^2.
+template <typename T>
+__global__ void __launch_bounds__(STAT_THREADS)
+k_spec_sums(Geom g, CPtrs<T> q, const double* __restrict__ kxv, const double* __restrict__ kyv,
+            const double* __restrict__ kzv, int S, double inn2, double* __restrict__ part) {
+  __shared__ double sh[SPEC_NQ * STAT_THREADS];
+  const int ix = blockIdx.x;
+  double acc[SPEC_NQ];
+  for (int qq = 0; qq < SPEC_NQ; ++qq) acc[qq] = 0.0;
+  const double kx = kxv[ix];
+  const long long n = (long long)g.ny * g.nzh;
+  for (long long i = threadIdx.x; i < n; i += STAT_THREADS) {
+    const int iy = (int)(i / g.nzh), iz = (int)(i % g.nzh);
+    const long long o = ((long long)ix * g.ny + iy) * g.kzs + iz;
+    const double ky = kyv[iy], kz = kzv[iz];
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double w = (iz == 0 || 2 * iz == g.nz) ? 1.0 : 2.0;
+    const cx<T> u = q.p[0][o], v = q.p[1][o], ww = q.p[2][o];
+    const double ux = u.x, uy = u.y, vx = v.x, vy = v.y, wx = ww.x, wy = ww.y;
+    const double e = (ux * ux + uy * uy + vx * vx + vy * vy + wx * wx + wy * wy) * inn2;
+    const double dr = kx * ux + ky * vx + kz * wx, di = kx * uy + ky * vy + kz * wy;
+    acc[0] += w * e;
+    acc[1] += w * k2 * e;
+    if (k2 > 0) acc[2] += w * e / (2.0 * sqrt(k2));
+    acc[3] += w * (dr * dr + di * di) * inn2;
+    for (int s = 0; s < S; ++s) {
+      const cx<T> ph = q.p[3 + s][o];
+      const double pr = ph.x, pi = ph.y;
+      const double ep = (pr * pr + pi * pi) * inn2;
+      acc[4 + 2 * s] += w * k2 * ep;
+      if (k2 > 0) acc[5 + 2 * s] += w * ep;
+    }
+  }
+  double tot[SPEC_NQ];
+  block_sum<SPEC_NQ>(acc, sh, tot);
+  if (threadIdx.x < SPEC_NQ) part[(long long)ix * SPEC_NQ + threadIdx.x] = tot[threadIdx.x];
+}
+
+struct ShellArgs {
+  int equal_box;        // 1: shell = nint(|m|) in integers
+  double fx, fy, fz;    // else shell = nint(sqrt((mx fx)^2 + (my fy)^2 + (mz fz)^2))
+};
+
+__device__ __forceinline__ long long isqrt_ll(long long v) {
+  long long r = (long long)sqrt((double)v);
+  while (r * r > v) --r;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return r;
+}
+__device__ __forceinline__ int shell_of(const ShellArgs& a, int mx, int my, int mz) {
+  if (a.equal_box) {
+    const long long m2 = (long long)mx * mx + (long long)my * my + (long long)mz * mz;
+    return (int)((isqrt_ll(4 * m2) + 1) / 2);
+  }
+  const double x = mx * a.fx, y = my * a.fy, z = mz * a.fz;
+  return (int)floor(sqrt(x * x + y * y + z * z) + 0.5);
+}
+
+// Shell spectrum of one kx plane (the reference declares it but never computes it,
+// statistics.cu:1847-1934).  Thread b owns shell b: for each ky it walks the kz interval of
+// that shell (the shell index is non-decreasing in kz), so every sum has a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(STAT_THREADS)
+k_spectrum(Geom g, CPtrs<T> q, int S, int nb, ShellArgs sa, double inn2, double* __restrict__ spec) {
+  const int ix = blockIdx.x;
+  const int mx = mode_of(ix, g.nx);
+  for (int b = threadIdx.x; b < nb; b += STAT_THREADS) {
+    double acc[1 + 4];
+    for (int s = 0; s <= S; ++s) acc[s] = 0.0;
+    for (int iy = 0; iy < g.ny; ++iy) {
+      const int my = mode_of(iy, g.ny);
+      if (shell_of(sa, mx, my, 0) > b) continue;
+      if (shell_of(sa, mx, my, g.nz / 2) < b) continue;
+      // first kz with shell >= b, first kz with shell >= b + 1
+      int lo = 0, hi = g.nzh;
+      while (lo < hi) { int mid = (lo + hi) >> 1; if (shell_of(sa, mx, my, mid) >= b) hi = mid; else lo = mid + 1; }
+      const int z0 = lo;
+      hi = g.nzh;
+      while (lo < hi) { int mid = (lo + hi) >> 1; if (shell_of(sa, mx, my, mid) >= b + 1) hi = mid; else lo = mid + 1; }
+      const int z1 = lo;
+      const long long base = ((long long)ix * g.ny + iy) * g.kzs;
+      for (int iz = z0; iz < z1; ++iz) {
+        const double w = (iz == 0 || 2 * iz == g.nz) ? 0.5 : 1.0;
+        const cx<T> u = q.p[0][base + iz], v = q.p[1][base + iz], ww = q.p[2][base + iz];
+        const double ux = u.x, uy = u.y, vx = v.x, vy = v.y, wx = ww.x, wy = ww.y;
+        acc[0] += w * (ux * ux + uy * uy + vx * vx + vy * vy + wx * wx + wy * wy) * inn2;
+        for (int s = 0; s < S; ++s) {
+          const cx<T> ph = q.p[3 + s][base + iz];
+          const double pr = ph.x, pi = ph.y;
+          acc[1 + s] += w * (pr * pr + pi * pi) * inn2;
+        }
+      }
+    }
+    for (int s = 0; s <= S; ++s) spec[((long long)ix * (1 + S) + s) * nb + b] = acc[s];
+  }
+}
+
+// x-z plane averages at one y (calcYprof_kernel_2D, statistics.cu:1936-1977), two passes:
+// means, then <u'u'>, <v'v'>, <w'w'>, <u'v'>, <phi'phi'>, <v'phi'>, plus the CFL and |u| maxima.
+template <typename T>
+__global__ void __launch_bounds__(STAT_THREADS)
+k_plane_stats(Geom g, RPtrs<T> ph, int S, double idx_, double idy_, double idz_,
+              double* __restrict__ prof, double* __restrict__ cmax) {
+  constexpr int NQ = 4 + 2 * 4;
+  __shared__ double sh[NQ * STAT_THREADS];
+  __shared__ double mean[3 + 4];
+  const int y = blockIdx.x;
+  const int F = 3 + S;
+  const long long n = (long long)g.nx * g.nz;
+  const double inv = 1.0 / (double)n;
+  double acc[NQ];
+  for (int qq = 0; qq < NQ; ++qq) acc[qq] = 0.0;
+  for (long long i = threadIdx.x; i < n; i += STAT_THREADS) {
+    const int x = (int)(i / g.nz), z = (int)(i % g.nz);
+    const long long o = ((long long)x * g.ny + y) * g.nz + z;
+    for (int f = 0; f < F; ++f) acc[f] += (double)ph.p[f][o];
+  }
+  double tot[NQ];
+  block_sum<NQ>(acc, sh, tot);
+  if (threadIdx.x < F) mean[threadIdx.x] = tot[threadIdx.x] * inv;
+  __syncthreads();
+  for (int qq = 0; qq < NQ; ++qq) acc[qq] = 0.0;
+  double cm = 0.0, um = 0.0;
+  for (long long i = threadIdx.x; i < n; i += STAT_THREADS) {
+    const int x = (int)(i / g.nz), z = (int)(i % g.nz);
+    const long long o = ((long long)x * g.ny + y) * g.nz + z;
+    const double u = ph.p[0][o], v = ph.p[1][o], w = ph.p[2][o];
+    const double du = u - mean[0], dv = v - mean[1], dw = w - mean[2];
+    acc[0] += du * du;
+    acc[1] += dv * dv;
+    acc[2] += dw * dw;
+    acc[3] += du * dv;
+    for (int s = 0; s < S; ++s) {
+      const double dp = (double)ph.p[3 + s][o] - mean[3 + s];
+      acc[4 + 2 * s] += dp * dp;
+      acc[5 + 2 * s] += dv * dp;
+    }
+    const double cf = fabs(u) * idx_ + fabs(v) * idy_ + fabs(w) * idz_;
+    cm = fmax(cm, cf);
+    um = fmax(um, fmax(fabs(u), fmax(fabs(v), fabs(w))));
+  }
+  block_sum<NQ>(acc, sh, tot);
+  // max reductions (order independent)
+  __syncthreads();
+  sh[threadIdx.x] = cm;
+  sh[STAT_THREADS + threadIdx.x] = um;
+  __syncthreads();
+  for (int s = STAT_THREADS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+      sh[STAT_THREADS + threadIdx.x] = fmax(sh[STAT_THREADS + threadIdx.x], sh[STAT_THREADS + threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int ny = g.ny;
+    prof[0 * ny + y] = mean[0];
+    prof[1 * ny + y] = mean[1];
+    prof[2 * ny + y] = mean[2];
+    prof[3 * ny + y] = tot[0] * inv;
+    prof[4 * ny + y] = tot[1] * inv;
+    prof[5 * ny + y] = tot[2] * inv;
+    prof[6 * ny + y] = tot[3] * inv;
+    for (int s = 0; s < S; ++s) {
+      prof[(7 + 3 * s) * ny + y] = mean[3 + s];
+      prof[(8 + 3 * s) * ny + y] = tot[4 + 2 * s] * inv;
+      prof[(9 + 3 * s) * ny + y] = tot[5 + 2 * s] * inv;
+    }
+    cmax[2 * y] = sh[0];
+    cmax[2 * y + 1] = sh[STAT_THREADS];
+  }
+}
+
+// Any non-finite value in the kz = 0 column of a field (cheap NaN/Inf watchdog after a step).
+template <typename T>
+__global__ void k_finite(Geom g, CPtrs<T> q, int F, int* __restrict__ flag) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)g.nx * g.ny) return;
+  for (int f = 0; f < F; ++f) {
+    const cx<T> v = q.p[f][i * g.kzs];
+    if (!isfinite((double)v.x) || !isfinite((double)v.y)) flag[0] = 1;
+  }
+}
+
+}  // namespace gnss

@@ -1,0 +1,109 @@
+"""Engine (libgnss_hip.so, MI355X) against the FP64 oracle on the same inputs.
+
+Bar: FP64 transforms <= 1e-13 relative; integer wavenumbers/masks bit-exact; initial condition
+identical to the last bit; FP64 steps rel-L2 <= 1e-12 on u, v, w, phi."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+SIZES = [(16, 16, 16), (24, 32, 48), (64, 64, 64), (96, 96, 96), (128, 64, 32), (32, 384, 16)]
+
+
+@pytest.mark.parametrize("n", SIZES, ids=lambda n: "x".join(map(str, n)))
+def test_forward_inverse_fp64(g, engine, oracle, n):
+    print(f"\n[fft fp64 {n}]", flush=True)
+    cfg = g.Config(n=n, sc=())
+    a, b = g.Solver(cfg, engine), g.Solver(cfg, oracle)
+    x = np.random.default_rng(sum(n)).standard_normal(n)
+    for s in (a, b):
+        s.set_physical(0, x)
+        s.forward(0)
+    fa, fb = a.get_spectral(0), b.get_spectral(0)
+    assert rel(fa, fb) < 1e-13
+    ref = np.fft.rfftn(x).transpose(1, 0, 2)
+    assert rel(fa, ref) < 1e-13
+    for s in (a, b):
+        s.inverse(0)
+    assert np.abs(a.get_physical(0) - x).max() < 1e-12
+    assert rel(a.get_physical(0), b.get_physical(0)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [(64, 64, 64), (96, 96, 96), (32, 384, 16)], ids=lambda n: "x".join(map(str, n)))
+def test_forward_inverse_fp32(g, engine, oracle, n):
+    print(f"\n[fft fp32 {n}]", flush=True)
+    a = g.Solver(g.Config(n=n, sc=(), precision=g.FP32), engine)
+    x = np.random.default_rng(7).standard_normal(n)
+    a.set_physical(0, x)
+    a.forward(0)
+    ref = np.fft.rfftn(x).transpose(1, 0, 2)
+    assert rel(a.get_spectral(0), ref) < 2e-6
+    a.inverse(0)
+    assert rel(a.get_physical(0), x) < 2e-6
+
+
+@pytest.mark.parametrize("n", [(64, 64, 64), (24, 96, 48)], ids=lambda n: "x".join(map(str, n)))
+def test_wavenumbers_and_mask_bit_exact(g, engine, oracle, n):
+    L = (2 * np.pi, 3 * np.pi, 2 * np.pi)
+    cfg = g.Config(n=n, L=L)
+    a, b = g.Solver(cfg, engine), g.Solver(cfg, oracle)
+    a.init_wavenumbers()
+    b.init_wavenumbers()
+    for ka, kb in zip(a.wavenumbers(), b.wavenumbers()):
+        assert ka.tobytes() == kb.tobytes()
+    assert a.mask().tobytes() == b.mask().tobytes()
+
+
+def test_initial_condition_identical(g, engine, oracle):
+    cfg = g.config("jet64")
+    a, b = g.Solver(cfg, engine).setup(), g.Solver(cfg, oracle).setup()
+    for f in range(4):
+        assert a.get_spectral(f).tobytes() == b.get_spectral(f).tobytes(), f"field {f}"
+
+
+@pytest.mark.parametrize("nsteps", [1, 5])
+def test_step_fp64_32(g, engine, oracle, nsteps):
+    print(f"\n[step fp64 32^3 x{nsteps}]", flush=True)
+    cfg = g.Config(n=(32, 32, 32), re=300.0, sc=(0.7, 2.0), theta=0.1, noise_kmax=6.0, noise_amp=0.2, dt=2e-3)
+    a, b = g.Solver(cfg, engine).setup(), g.Solver(cfg, oracle).setup()
+    a.step(nsteps)
+    b.step(nsteps)
+    for f in range(5):
+        assert rel(a.get_spectral(f), b.get_spectral(f)) < 1e-12, f
+
+
+def test_jet64_100_steps_fp64(g, engine, oracle):
+    """Plain FP64 temporal jet at 64^3, 100 steps: rel L2 <= 1e-12 on u, v, w, phi."""
+    print("\n[jet64 fp64 x100]", flush=True)
+    cfg = g.config("jet64")
+    a, b = g.Solver(cfg, engine).setup(), g.Solver(cfg, oracle).setup()
+    for chunk in range(5):
+        a.step(20)
+        b.step(20)
+        errs = [rel(a.get_spectral(f), b.get_spectral(f)) for f in range(4)]
+        print(f"  after {20 * (chunk + 1)} steps: {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < 1e-12
+    sa, ya, pa = a.stats()
+    sb, yb, pb = b.stats()
+    for k in ["ke", "epsilon", "enstrophy", "lint", "cfl", "umax"]:
+        assert abs(sa[k] - sb[k]) <= 1e-10 * abs(sb[k]), k
+    assert abs(sa["chi"][0] - sb["chi"][0]) <= 1e-10 * sb["chi"][0]
+    assert np.abs(ya - yb).max() <= 1e-10 * np.abs(yb).max()
+    assert np.abs(pa - pb).max() <= 1e-10 * np.abs(pb).max()
+
+
+def test_stats_fp64(g, engine, oracle):
+    cfg = g.Config(n=(48, 32, 64), sc=(0.7,), noise_kmax=5.0, noise_amp=0.1)
+    a, b = g.Solver(cfg, engine).setup(), g.Solver(cfg, oracle).setup()
+    sa, ya, pa = a.stats()
+    sb, yb, pb = b.stats()
+    for k in ["ke", "vrms", "epsilon", "enstrophy", "lint", "lambda", "eta", "cfl", "umax"]:
+        assert abs(sa[k] - sb[k]) <= 1e-12 * abs(sb[k]), (k, sa[k], sb[k])
+    assert abs(sa["divergence"]) < 1e-13
+    assert np.abs(ya - yb).max() <= 1e-12 * np.abs(yb).max()
+    assert np.abs(pa - pb).max() <= 1e-12 * np.abs(pb).max()
