@@ -127,7 +127,7 @@ def main():
     print(f"[bench] {args.config} FP32 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
     s32, t32, rep = run_gpu(cfg32, args.steps, args.warmup, timing=True)
     print(f"[bench] FP32 {args.steps / t32:.3f} steps/s", file=sys.stderr, flush=True)
-    s64, t64, _ = run_gpu(cfg64, args.steps, args.warmup, timing=False)
+    s64, t64, rep64 = run_gpu(cfg64, args.steps, args.warmup, timing=True)
     print(f"[bench] FP64 {args.steps / t64:.3f} steps/s", file=sys.stderr, flush=True)
     errs = {}
     names = ["u", "v", "w"] + [f"phi{i}" for i in range(len(cfg32.sc))]
@@ -170,6 +170,7 @@ def main():
                                "after_steps": args.warmup + args.steps},
         "fft3d": {"f32": fft32, "f64": fft64},
         "kernels_f32": rep,
+        "kernels_f64": rep64,
         "roofline": roof,
     }
     if not args.no_cpu:

@@ -8,6 +8,8 @@
 
 #include <string>
 
+#include <dlfcn.h>
+
 #include "engine.h"
 
 #define GNSS_API extern "C" __attribute__((visibility("default")))
@@ -95,6 +97,16 @@ KernelTimer::~KernelTimer() {
     (void)hipEventDestroy(r.b);
   }
   for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+std::string library_dir() {
+  Dl_info info;
+  if (dladdr((void*)&library_dir, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    const size_t k = p.find_last_of('/');
+    return k == std::string::npos ? "." : p.substr(0, k);
+  }
+  return ".";
 }
 
 }  // namespace gnss
@@ -225,3 +237,17 @@ GNSS_API int gnss_bench_fft(gnss_ctx* c, int reps, double* ms_fwd, double* ms_in
 // ---------------------------------------------------------------------------------------------
 // Ensemble entry points are implemented in ensemble.hip.
 // ---------------------------------------------------------------------------------------------
+
+GNSS_API int gnss_autotune(gnss_ctx* c, int reps, char* report, int len) {
+  return guard(c, [&] {
+    std::string s = c->eng->autotune(reps);
+    if (report && len > 0) {
+      if ((int)s.size() + 1 > len) throw Error(GNSS_ERR_ARG, "report buffer too small");
+      memcpy(report, s.c_str(), s.size() + 1);
+    }
+  });
+}
+GNSS_API int gnss_set_tuning(gnss_ctx* c, const char* spec) {
+  if (!spec) return GNSS_ERR_ARG;
+  return guard(c, [&] { c->eng->set_tuning(spec); });
+}

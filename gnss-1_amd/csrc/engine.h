@@ -72,6 +72,15 @@ class EngineBase {
   virtual void timing_enable(bool on) = 0;
   virtual std::string timing_report() = 0;
   virtual void bench_fft(int reps, double* ms_fwd, double* ms_inv) = 0;
+  virtual std::string autotune(int reps) = 0;
+  virtual void set_tuning(const std::string& spec) = 0;
+  // Ensemble support: the stream the engine launches on, the kernels of one RK3 step without any
+  // host synchronisation, memset or allocation (capturable into a hipGraph), the host-side clock
+  // of a replayed step, and the NaN/Inf watchdog.
+  virtual hipStream_t stream() const = 0;
+  virtual void enqueue_rk3_step() = 0;
+  virtual void advance_clock(int nsteps) = 0;
+  virtual void check_finite() = 0;
   std::string err;
 };
 
@@ -85,6 +94,9 @@ inline bool supported_len(int n) {
       return false;
   }
 }
+
+// Directory of the loaded engine library (for its tuning table).
+std::string library_dir();
 
 EngineBase* make_engine_f64(const gnss_params& p);
 EngineBase* make_engine_f32(const gnss_params& p);

@@ -38,7 +38,9 @@ class Engine : public EngineBase {
     W_ = 6 + 3 * S_;
     cx_ = col_chunk(nx_);
     cy_ = col_chunk(ny_);
-    const int cmax = std::max(cx_, cy_);
+    zf_ = 0;
+    // kz padding and the pruned kz limit use the widest column tile any variant may choose
+    const int cmax = sizeof(T) == 8 ? 8 : 16;
     kzs_ = round_up(nzh_, cmax);
     g_.nx = nx_; g_.ny = ny_; g_.nz = nz_; g_.nzh = nzh_; g_.kzs = kzs_;
     g_.keep_x = nx_ / 3; g_.keep_y = ny_ / 3; g_.keep_z = nz_ / 3;
@@ -48,6 +50,7 @@ class Engine : public EngineBase {
     kz_lim_ = std::min(round_up(g_.keep_z + 1, cmax), kzs_);
     nsp_ = (long long)nx_ * ny_ * kzs_;
     nph_ = (long long)nx_ * ny_ * nz_;
+    load_tuning();
     GNSS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     try {
       allocate();
@@ -161,12 +164,25 @@ class Engine : public EngineBase {
     timed("k_dealias", (double)nsp_ * F_ * sizeof(C), [&] {
       hipLaunchKernelGGL(k_dealias<T>, dim3(cdiv(nsp_, 256)), dim3(256), 0, s_, g_, ptrs(q_), F_, nsp_);
     });
-    for (int n = 0; n < nsteps; ++n) {
-      for (int s = 0; s < 3; ++s) substage(s);
-      step_ += 1;
-      t_ = (double)step_ * p_.dt;
-    }
-    // NaN/Inf watchdog
+    for (int n = 0; n < nsteps; ++n) enqueue_rk3_step();
+    advance_clock(nsteps);
+    check_finite();
+  }
+
+  hipStream_t stream() const override { return s_; }
+
+  void enqueue_rk3_step() override {
+    need_k();
+    for (int s = 0; s < 3; ++s) substage(s);
+  }
+
+  void advance_clock(int nsteps) override {
+    step_ += nsteps;
+    t_ = (double)step_ * p_.dt;
+  }
+
+  // NaN/Inf watchdog on the kz = 0 column of every field.
+  void check_finite() override {
     GNSS_HIP(hipMemsetAsync(d_flag_, 0, sizeof(int), s_));
     hipLaunchKernelGGL(k_finite<T>, dim3(cdiv((long long)nx_ * ny_, 256)), dim3(256), 0, s_, g_,
                        ptrs(q_), F_, d_flag_);
@@ -345,7 +361,126 @@ class Engine : public EngineBase {
     if (ms_inv) *ms_inv = ti / reps;
   }
 
+ public:
+  // -------------------------------------------------------------------------------------------
+  // Autotuner: times the compiled variants of the step's passes on this context's buffers and
+  // keeps the fastest (column tile width per strided axis, plan of the fused z kernel).
+  // ------------------------------------------------------------------------------------------
+  std::string tuning_key() const {
+    char b[96];
+    snprintf(b, sizeof b, "%s %d %d %d %d", sizeof(T) == 8 ? "f64" : "f32", nx_, ny_, nz_, S_);
+    return b;
+  }
+  std::string tuning_string() const {
+    char b[64];
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d", cx_, cy_, zf_);
+    return b;
+  }
+  void set_tuning(const std::string& spec) override {
+    int cx = cx_, cy = cy_, zf = zf_;
+    const char* p = spec.c_str();
+    const char* q;
+    if ((q = strstr(p, "cx=")) || (q = strstr(p, "\"cx\": "))) cx = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "cy=")) || (q = strstr(p, "\"cy\": "))) cy = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 1))
+      throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
+    cx_ = cx;
+    cy_ = cy;
+    zf_ = zf;
+  }
+
+  std::string autotune(int reps) override {
+    need_k();
+    if (reps <= 0) return "{\"key\": \"" + tuning_key() + "\", \"choice\": \"" + tuning_string() + "\"}";
+    const bool was_on = timer_.on();
+    timer_.enable(false, s_);
+    hipEvent_t e0, e1;
+    GNSS_HIP(hipEventCreate(&e0));
+    GNSS_HIP(hipEventCreate(&e1));
+    auto time_it = [&](auto&& fn) {
+      fn();  // warm-up
+      GNSS_HIP(hipEventRecord(e0, s_));
+      for (int r = 0; r < reps; ++r) fn();
+      GNSS_HIP(hipEventRecord(e1, s_));
+      GNSS_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      GNSS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / reps;
+    };
+    std::string rep = "{\"key\": \"" + tuning_key() + "\", \"candidates\": {";
+    char buf[160];
+    bool first = true;
+    // x and y passes as the step runs them (pruned inverse then forward), per tile width
+    for (int axis = 0; axis < 2; ++axis) {
+      const int L = axis == 0 ? nx_ : ny_;
+      int best = axis == 0 ? cx_ : cy_;
+      double best_ms = 1e30;
+      for (int c : {4, 8, 16}) {
+        if (!chunk_ok(L, c)) continue;
+        (axis == 0 ? cx_ : cy_) = c;
+        const double ms = time_it([&] {
+          if (axis == 0) {
+            col_pass(ws_[0], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+            col_pass(ws_[0], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
+          } else {
+            col_pass(ws_[0], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+            col_pass(ws_[0], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
+          }
+        });
+        snprintf(buf, sizeof buf, "%s\"c%c=%d\": %.5f", first ? "" : ", ", axis == 0 ? 'x' : 'y', c, ms);
+        rep += buf;
+        first = false;
+        if (ms < best_ms) { best_ms = ms; best = c; }
+      }
+      (axis == 0 ? cx_ : cy_) = best;
+    }
+    {
+      int best = zf_;
+      double best_ms = 1e30;
+      for (int v = 0; v <= (sizeof(T) == 8 ? 0 : 1); ++v) {
+        zf_ = v;
+        const double ms = time_it([&] { z_fused(); });
+        snprintf(buf, sizeof buf, ", \"zf=%d\": %.5f", v, ms);
+        rep += buf;
+        if (ms < best_ms) { best_ms = ms; best = v; }
+      }
+      zf_ = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    timer_.enable(was_on, s_);
+    return rep + "}, \"choice\": \"" + tuning_string() + "\"}";
+  }
+
  private:
+  bool chunk_ok(int L, int c) const {
+    if (sizeof(T) == 8 ? (c != 4 && c != 8) : (c != 8 && c != 16)) return false;
+    return (long long)L * c * (long long)sizeof(C) <= LDS_MAX;
+  }
+
+  // Table of tuned choices: one line per key, "<key>": {"cx": C, "cy": C, "zf": V}.  Looked up in
+  // $GNSS_TUNING or tuning/gnss_tuning.json next to the library; absent entries keep defaults.
+  void load_tuning() {
+    std::string path;
+    if (const char* e = getenv("GNSS_TUNING")) path = e;
+    else path = library_dir() + "/../tuning/gnss_tuning.json";
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return;
+    const std::string key = "\"" + tuning_key() + "\"";
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+      if (strstr(line, key.c_str())) {
+        try {
+          set_tuning(strchr(line, '{') ? strchr(line, '{') : line);
+        } catch (const Error&) {
+        }
+        break;
+      }
+    }
+    fclose(f);
+  }
+
   // -------------------------------------------------------------------------------------------
   int col_chunk(int L) const {
     int c = sizeof(T) == 8 ? 8 : 16;
@@ -695,20 +830,70 @@ class Engine : public EngineBase {
     return b;
   }
 
+  template <int NZ>
+  void zfused_launch(const ZFusedArgs<T>& a0, T scale) {
+    if constexpr (sizeof(T) == 4) {
+      if (zf_ == 1) return zfused_launch_v<NZ, 1>(a0, scale);
+    }
+    zfused_launch_v<NZ, 0>(a0, scale);
+  }
+
+  template <int NZ, int V>
+  void zfused_launch_v(const ZFusedArgs<T>& a0, T scale) {
+    constexpr int M = NZ / 2;
+    using P = typename FusedPlan<T, M, V>::type;
+    constexpr int TPC = M / P::E;
+    int G = std::max(1, 256 / TPC);
+    const long long row_lds = (long long)FUSED_B * zrow_stride(M) * (long long)sizeof(C);
+    while (G > 1 && G * row_lds > LDS_MAX) --G;
+    ZFusedArgs<T> a = a0;
+    a.G = G;
+    hipLaunchKernelGGL((k_zfused<T, NZ, V>), dim3((unsigned)cdiv(a.nrows, G)), dim3(TPC * G),
+                       (size_t)(G * row_lds), s_, a, tw_.at(M), tw_.at(NZ), scale);
+  }
+
+  // z pass of a substage fused with the physical products: ws (after the x, y inverse passes)
+  // -> nl (before the y, x forward passes), kz < kz_lim_.
+  void z_fused() {
+    ZFusedArgs<T> a{};
+    for (int w = 0; w < W_; ++w) a.ws[w] = ws_[w];
+    for (int f = 0; f < F_; ++f) a.nl[f] = nl_[f];
+    a.nrows = (long long)nx_ * ny_;
+    a.kzs = kzs_;
+    a.lim = kz_lim_;
+    a.S = S_;
+    const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
+    const double bytes = (double)nx_ * ny_ * (double)kz_lim_ * sizeof(C) * (W_ + F_);
+    timed("fft_z_fused", bytes, [&] { GNSS_Z_SWITCH(zfused_launch, a, scale); });
+  }
+
   void substage(int s) {
-    ensure_unfused_buffers();
+    const bool unfused = (p_.flags & GNSS_FLAG_UNFUSED) != 0;
     const Box b = kept_box();
     const long long nb = (long long)b.bx * b.by * b.bz;
     timed("k_prologue", (double)nb * (F_ + W_) * sizeof(C), [&] {
       hipLaunchKernelGGL(k_prologue<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_),
                          ptrs(ws_), tkx_, tky_, tkz_, S_, nb);
     });
-    for (int w = 0; w < W_; ++w) inv3d(ws_[w], wp_[w], true);
-    timed("k_products", (double)nph_ * (W_ + F_) * sizeof(T), [&] {
-      hipLaunchKernelGGL(k_products<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, rptrs(wp_),
-                         rptrs(nlp_), S_, nph_);
-    });
-    for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
+    if (unfused) {
+      ensure_unfused_buffers();
+      for (int w = 0; w < W_; ++w) inv3d(ws_[w], wp_[w], true);
+      timed("k_products", (double)nph_ * (W_ + F_) * sizeof(T), [&] {
+        hipLaunchKernelGGL(k_products<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, rptrs(wp_),
+                           rptrs(nlp_), S_, nph_);
+      });
+      for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
+    } else {
+      for (int w = 0; w < W_; ++w) {
+        col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+        col_pass(ws_[w], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+      }
+      z_fused();
+      for (int f = 0; f < F_; ++f) {
+        col_pass(nl_[f], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
+        col_pass(nl_[f], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
+      }
+    }
     RkCoef c;
     c.a = RK_A[s];
     c.b = RK_B[s];
@@ -729,7 +914,7 @@ class Engine : public EngineBase {
 
   // -------------------------------------------------------------------------------------------
   gnss_params p_;
-  int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8;
+  int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

@@ -174,61 +174,110 @@ struct LdsRow {
   __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
 };
 
-// One Stockham stage with radix R after a span NS; recursion over the remaining radices.
-template <typename T, int L, int E, int SIGN, int NS, typename Idx, int R, int... Rest>
+// Synchronisation between radix stages: a whole block, or only the calling wavefront when every
+// transform lives inside one wavefront (its LDS traffic is then ordered by the wave itself and
+// only compiler reordering has to be prevented).
+template <bool WAVE> __device__ __forceinline__ void lds_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// One Stockham stage with radix R after a span NS, applied to B independent transforms at once
+// (transform b uses LDS region idx(pos) + b * BSTRIDE); recursion over the remaining radices.
+// Batching lets one LDS round trip serve B transforms, so the exchange latency is paid once.
+template <typename T, int L, int E, int SIGN, int NS, typename Idx, bool WAVE, int B, int BSTRIDE, int R,
+          int... Rest>
 struct Stages {
-  __device__ __forceinline__ static void run(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
+  __device__ __forceinline__ static void run(cx<T> (*v)[E], cx<T>* lds, int t, const Idx& idx,
                                              const cx<T>* __restrict__ tw) {
     constexpr int TPC = L / E;
     constexpr int NB = E / R;
     constexpr bool LAST = sizeof...(Rest) == 0;
     static_assert(E % R == 0, "radix must divide the elements per thread");
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int j = t + b * TPC;
+    for (int bt = 0; bt < NB; ++bt) {
+      const int j = t + bt * TPC;
       const int k = (NS == 1) ? 0 : (j % NS);
-      cx<T> a[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
+      cx<T> w[R];
       if (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) a[r] = cmul(a[r], twid<SIGN>(tw, r * k * (L / (NS * R))));
+        for (int r = 1; r < R; ++r) w[r] = twid<SIGN>(tw, r * k * (L / (NS * R)));
       }
-      Dft<R, SIGN, cx<T>>::run(a);
-      if (LAST) {
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[b + q * NB] = a[q];
-      } else {
-        const int base = (j / NS) * NS * R + k;
+      for (int b = 0; b < B; ++b) {
+        cx<T> a[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) lds[idx(base + q * NS)] = a[q];
+        for (int r = 0; r < R; ++r) a[r] = v[b][bt + r * NB];
+        if (NS > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+        }
+        Dft<R, SIGN, cx<T>>::run(a);
+        if (LAST) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) v[b][bt + q * NB] = a[q];
+        } else {
+          const int base = (j / NS) * NS * R + k;
+#pragma unroll
+          for (int q = 0; q < R; ++q) lds[idx(base + q * NS) + b * BSTRIDE] = a[q];
+        }
       }
     }
     if constexpr (!LAST) {
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + e * TPC)];
-      __syncthreads();
-      Stages<T, L, E, SIGN, NS * R, Idx, Rest...>::run(v, lds, t, idx, tw);
+      for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[b][e] = lds[idx(t + e * TPC) + b * BSTRIDE];
+      lds_sync<WAVE>();
+      Stages<T, L, E, SIGN, NS * R, Idx, WAVE, B, BSTRIDE, Rest...>::run(v, lds, t, idx, tw);
     }
   }
 };
 
-template <typename T, int L, int SIGN, typename Idx, typename Rad> struct FftL;
-template <typename T, int L, int SIGN, typename Idx, int... Rs>
-struct FftL<T, L, SIGN, Idx, Radices<Rs...>> {
-  __device__ __forceinline__ static void run(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
+// Plans for kernels that keep several fields in registers at once: the fewest elements per
+// thread that still fit one transform into a 64-lane wavefront.  Lengths with a factor 3 use the
+// default plan (at most 32 threads per transform).
+template <int L> struct PlanLow : Plan<L> {};
+template <> struct PlanLow<8> { static constexpr int E = 4; using R = Radices<4, 2>; };
+template <> struct PlanLow<16> { static constexpr int E = 4; using R = Radices<4, 4>; };
+template <> struct PlanLow<32> { static constexpr int E = 4; using R = Radices<4, 4, 2>; };
+template <> struct PlanLow<64> { static constexpr int E = 4; using R = Radices<4, 4, 4>; };
+template <> struct PlanLow<128> { static constexpr int E = 4; using R = Radices<4, 4, 4, 2>; };
+template <> struct PlanLow<256> { static constexpr int E = 4; using R = Radices<4, 4, 4, 4>; };
+template <> struct PlanLow<512> { static constexpr int E = 8; using R = Radices<8, 8, 8>; };
+
+template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int B, int BSTRIDE, typename Rad>
+struct FftL;
+template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int B, int BSTRIDE, int... Rs>
+struct FftL<T, L, SIGN, Idx, P, WAVE, B, BSTRIDE, Radices<Rs...>> {
+  __device__ __forceinline__ static void run(cx<T> (*v)[P::E], cx<T>* lds, int t, const Idx& idx,
                                              const cx<T>* __restrict__ tw) {
-    Stages<T, L, Plan<L>::E, SIGN, 1, Idx, Rs...>::run(v, lds, t, idx, tw);
+    Stages<T, L, P::E, SIGN, 1, Idx, WAVE, B, BSTRIDE, Rs...>::run(v, lds, t, idx, tw);
   }
 };
 
-// Transform of length L held in strided order by TPC = L / Plan<L>::E threads.
-// Contains __syncthreads(): every thread of the block must call it.
-template <typename T, int L, int SIGN, typename Idx>
+// Transform of length L held in strided order by TPC = L / P::E threads.  With WAVE = false it
+// contains __syncthreads() (every thread of the block must call it); with WAVE = true the TPC
+// threads of one transform must belong to one wavefront.
+template <typename T, int L, int SIGN, typename Idx, typename P = Plan<L>, bool WAVE = false>
 __device__ __forceinline__ void fft(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
                                     const cx<T>* __restrict__ tw) {
-  FftL<T, L, SIGN, Idx, typename Plan<L>::R>::run(v, lds, t, idx, tw);
+  static_assert(!WAVE || (L / P::E) <= 64, "a wave-synchronised transform must fit one wavefront");
+  FftL<T, L, SIGN, Idx, P, WAVE, 1, 0, typename P::R>::run(reinterpret_cast<cx<T>(*)[P::E]>(v), lds, t, idx, tw);
+}
+
+// B transforms at once; transform b exchanges through LDS region idx(pos) + b * BSTRIDE.
+template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int B, int BSTRIDE>
+__device__ __forceinline__ void fft_batch(cx<T> (*v)[P::E], cx<T>* lds, int t, const Idx& idx,
+                                          const cx<T>* __restrict__ tw) {
+  static_assert(!WAVE || (L / P::E) <= 64, "a wave-synchronised transform must fit one wavefront");
+  FftL<T, L, SIGN, Idx, P, WAVE, B, BSTRIDE, typename P::R>::run(v, lds, t, idx, tw);
 }
 
 // Signed mode index of storage index i along an axis of length n.

@@ -83,49 +83,144 @@ k_col(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a) {
 // Row stride (complex entries) of the LDS tile used by the z passes: M+1 values plus padding.
 __host__ __device__ constexpr int zrow_stride(int M) { return M + 2 + (M >> 4); }
 
+// One contiguous z row held by TPC threads as E complex values per thread.  The real row of NZ
+// values is packed as M = NZ/2 complex pairs (z = 2n, 2n+1); the half-length transform is
+// unpacked with exp(-2 pi i k / NZ) (R2C) or packed the same way before it (C2R).  TPC divides
+// 64, so a row never straddles wavefronts and all row-local LDS traffic is wave-synchronised.
+template <typename T, int NZ, typename P>
+struct ZRow {
+  static constexpr int M = NZ / 2;
+  static constexpr int E = P::E;
+  static constexpr int TPC = M / E;
+  static constexpr int RS = zrow_stride(M);
+  static_assert(64 % TPC == 0, "a z row must lie inside one wavefront");
+
+  // Registers of one spectral row as loaded from HBM (entries k >= lim read as 0).
+  struct Raw {
+    cx<T> x[E];
+    cx<T> xm;  // entry k = M (thread t == 0)
+  };
+
+  __device__ __forceinline__ static void load(const cx<T>* __restrict__ src, bool active, int lim, int t,
+                                              Raw& r) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + e * TPC;
+      r.x[e] = (active && k < lim) ? src[k] : mk<T>(0, 0);
+    }
+    r.xm = (active && t == 0 && M < lim) ? src[M] : mk<T>(0, 0);
+  }
+
+  // Value of register `e` in lane TPC - t of this row (t = 0 reads its own lane): the partner
+  // index M - k of k = t + e TPC lives there in register E - 1 - e.
+  __device__ __forceinline__ static cx<T> mirror(const cx<T>& x, int t) {
+    const int src = (TPC - t) & (TPC - 1);
+    cx<T> r;
+    r.x = __shfl(x.x, src, TPC);
+    r.y = __shfl(x.y, src, TPC);
+    return r;
+  }
+
+  // C2R of B loaded rows to packed real rows, scaled by `scale` (Im of k = 0 and k = M ignored).
+  // The pairing of k with M - k is a lane shuffle; only the radix stages use LDS, and the B rows
+  // go through each stage together (LDS region of row b at idx + b * RS).
+  template <int B>
+  __device__ __forceinline__ static void c2r(const Raw* r, cx<T>* lds, const LdsRow& idx, int t,
+                                             const cx<T>* __restrict__ twM,
+                                             const cx<T>* __restrict__ twN, T scale, cx<T> (*v)[E]) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      cx<T> x[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = r[b].x[e];
+      if (t == 0) x[0].y = 0;
+      cx<T> xc[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) xc[e] = mirror(x[E - 1 - e], t);
+      if (t == 0) {
+        xc[0] = mk<T>(r[b].xm.x, (T)0);
+#pragma unroll
+        for (int e = 1; e < E; ++e) xc[e] = x[E - e];
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = t + e * TPC;
+        const cx<T> xk = x[e];
+        const cx<T> c = cconj(xc[e]);
+        const cx<T> p = cmul(cconj(twN[k]), csub(xk, c));
+        cx<T> z;
+        z.x = xk.x + c.x - p.y;
+        z.y = xk.y + c.y + p.x;
+        v[b][e] = z;
+      }
+    }
+    lds_sync<true>();
+    fft_batch<T, M, +1, LdsRow, P, true, B, RS>(v, lds, t, idx, twM);
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[b][e] = cscale(v[b][e], scale);
+  }
+
+  // Transform B packed real rows to half spectra; entries k < lim of row b are stored to dst[b]
+  // (the Nyquist value at k = M and zero padding up to kzs included).
+  template <int B>
+  __device__ __forceinline__ static void r2c(cx<T> (*v)[E], cx<T>* const* dst, bool active, int lim,
+                                             int kzs, cx<T>* lds, const LdsRow& idx, int t,
+                                             const cx<T>* __restrict__ twM,
+                                             const cx<T>* __restrict__ twN) {
+    lds_sync<true>();
+    fft_batch<T, M, -1, LdsRow, P, true, B, RS>(v, lds, t, idx, twM);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      cx<T> zc[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) zc[e] = mirror(v[b][E - 1 - e], t);
+      if (t == 0) {
+        zc[0] = v[b][0];
+#pragma unroll
+        for (int e = 1; e < E; ++e) zc[e] = v[b][E - e];
+      }
+      if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int k = t + e * TPC;
+          if (k < lim) {
+            const cx<T> zk = v[b][e];
+            const cx<T> c = cconj(zc[e]);
+            const cx<T> p = cmul(twN[k], csub(zk, c));
+            cx<T> x;
+            x.x = (T)0.5 * (zk.x + c.x + p.y);
+            x.y = (T)0.5 * (zk.y + c.y - p.x);
+            dst[b][k] = x;
+          }
+        }
+        if (t == 0 && M < lim) dst[b][M] = mk<T>(v[b][0].x - v[b][0].y, (T)0);
+        for (int k = M + 1 + t; k < kzs && k < lim; k += TPC) dst[b][k] = mk<T>(0, 0);
+      }
+    }
+  }
+};
+
 // R2C along contiguous z rows: in = rows of NZ reals, out = rows of kzs complex (kz < out_lim
 // stored; entries between nz/2 and kzs are zero padding).
 template <typename T, int NZ>
 __global__ void __launch_bounds__(lb_threads((NZ / 2) / Plan<NZ / 2>::E))
 k_r2c_z(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ twM,
         const cx<T>* __restrict__ twN, long long nrows, int kzs, int out_lim, int G) {
-  constexpr int M = NZ / 2;
-  constexpr int E = Plan<M>::E;
-  constexpr int TPC = M / E;
-  constexpr int RS = zrow_stride(M);
+  using Z = ZRow<T, NZ, Plan<NZ / 2>>;
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int tid = threadIdx.x;
-  const int t = tid % TPC;
-  const int g = tid / TPC;
+  const int t = threadIdx.x % Z::TPC;
+  const int g = threadIdx.x / Z::TPC;
   const long long row = (long long)blockIdx.x * G + g;
   const bool active = row < nrows;
   const cx<T>* src = reinterpret_cast<const cx<T>*>(in + (active ? row : 0) * NZ);
-  cx<T> v[E];
+  cx<T> v[1][Z::E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = active ? src[t + e * TPC] : mk<T>(0, 0);
-  const LdsRow idx{g * RS};
-  fft<T, M, -1>(v, lds, t, idx, twM);
-#pragma unroll
-  for (int e = 0; e < E; ++e) lds[idx(t + e * TPC)] = v[e];
-  __syncthreads();
-  if (!active) return;
-  cx<T>* dst = out + row * kzs;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = t + e * TPC;
-    if (k < out_lim) {
-      const cx<T> zk = v[e];
-      const cx<T> zc = cconj(lds[idx(k == 0 ? 0 : M - k)]);
-      const cx<T> p = cmul(twN[k], csub(zk, zc));
-      cx<T> x;
-      x.x = (T)0.5 * (zk.x + zc.x + p.y);
-      x.y = (T)0.5 * (zk.y + zc.y - p.x);
-      dst[k] = x;
-    }
-  }
-  if (t == 0 && M < out_lim) dst[M] = mk<T>(v[0].x - v[0].y, (T)0);
-  for (int k = M + 1 + t; k < kzs && k < out_lim; k += TPC) dst[k] = mk<T>(0, 0);
+  for (int e = 0; e < Z::E; ++e) v[0][e] = active ? src[t + e * Z::TPC] : mk<T>(0, 0);
+  cx<T>* dst[1] = {out + (active ? row : 0) * kzs};
+  Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, twM, twN);
 }
 
 // C2R along contiguous z rows: in = rows of kzs complex (entries kz >= in_lim are treated as 0;
@@ -134,50 +229,118 @@ template <typename T, int NZ>
 __global__ void __launch_bounds__(lb_threads((NZ / 2) / Plan<NZ / 2>::E))
 k_c2r_z(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __restrict__ twM,
         const cx<T>* __restrict__ twN, long long nrows, int kzs, int in_lim, T scale, int G) {
-  constexpr int M = NZ / 2;
-  constexpr int E = Plan<M>::E;
-  constexpr int TPC = M / E;
-  constexpr int RS = zrow_stride(M);
+  using Z = ZRow<T, NZ, Plan<NZ / 2>>;
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int tid = threadIdx.x;
-  const int t = tid % TPC;
-  const int g = tid / TPC;
+  const int t = threadIdx.x % Z::TPC;
+  const int g = threadIdx.x / Z::TPC;
   const long long row = (long long)blockIdx.x * G + g;
   const bool active = row < nrows;
-  const cx<T>* src = in + (active ? row : 0) * kzs;
-  const LdsRow idx{g * RS};
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = t + e * TPC;
-    cx<T> x = (active && k < in_lim) ? src[k] : mk<T>(0, 0);
-    if (k == 0) x.y = 0;
-    lds[idx(k)] = x;
-  }
-  if (t == 0) {
-    cx<T> x = (active && M < in_lim) ? src[M] : mk<T>(0, 0);
-    x.y = 0;
-    lds[idx(M)] = x;
-  }
-  __syncthreads();
-  cx<T> v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = t + e * TPC;
-    const cx<T> xk = lds[idx(k)];
-    const cx<T> xc = cconj(lds[idx(M - k)]);
-    const cx<T> p = cmul(cconj(twN[k]), csub(xk, xc));
-    cx<T> z;
-    z.x = xk.x + xc.x - p.y;
-    z.y = xk.y + xc.y + p.x;
-    v[e] = z;
-  }
-  __syncthreads();
-  fft<T, M, +1>(v, lds, t, idx, twM);
+  typename Z::Raw r[1];
+  Z::load(in + (active ? row : 0) * kzs, active, in_lim, t, r[0]);
+  cx<T> v[1][Z::E];
+  Z::template c2r<1>(r, lds, LdsRow{g * Z::RS}, t, twM, twN, scale, v);
   if (!active) return;
   cx<T>* dst = reinterpret_cast<cx<T>*>(out + row * NZ);
 #pragma unroll
-  for (int e = 0; e < E; ++e) dst[t + e * TPC] = cscale(v[e], scale);
+  for (int e = 0; e < Z::E; ++e) dst[t + e * Z::TPC] = v[0][e];
+}
+
+// Plans of the fused z kernel (chosen by the autotuner): V = 0 the fewest elements per thread
+// (bounded registers), V = 1 the default plan (fewer radix stages, more registers).
+template <typename T, int M, int V> struct FusedPlan { using type = PlanLow<M>; };
+template <typename T, int M> struct FusedPlan<T, M, 1> { using type = Plan<M>; };
+
+template <typename T> struct ZFusedArgs {
+  const cx<T>* ws[18];   // u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar
+  cx<T>* nl[7];          // (u x omega)_x,y,z, u . grad phi per scalar
+  long long nrows;
+  int kzs, lim, S, G;
+};
+
+template <typename V> __device__ __forceinline__ void fma_pair(V& acc, const V& a, const V& b) {
+  acc.x += a.x * b.x;
+  acc.y += a.y * b.y;
+}
+template <typename V> __device__ __forceinline__ void fms_pair(V& acc, const V& a, const V& b) {
+  acc.x -= a.x * b.x;
+  acc.y -= a.y * b.y;
+}
+template <typename V> __device__ __forceinline__ V mul_pair(const V& a, const V& b) {
+  V r;
+  r.x = a.x * b.x;
+  r.y = a.y * b.y;
+  return r;
+}
+
+// Fused z pass of one substage: C2R of the 6 + 3S work rows, the physical products u x omega and
+// u . grad phi (crossProductKernel_mgpu / dotKernel_mgpu, solver.cu:139-164, 342-356), and R2C
+// of the 3 + S products — the physical fields never leave registers.  Rows hold kz < lim only.
+// Fields move through the radix stages three at a time (u, v, w; omega; products; grad phi),
+// and in FP32 the HBM loads of the next three rows are issued before the current ones transform.
+// Each row owns FUSED_B LDS regions of RS entries.
+constexpr int FUSED_B = 3;
+template <typename T, int NZ, int V>
+__global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E))
+k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, T scale) {
+  using Z = ZRow<T, NZ, typename FusedPlan<T, NZ / 2, V>::type>;
+  constexpr int E = Z::E;
+  constexpr bool PREFETCH = sizeof(T) == 4;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int t = threadIdx.x % Z::TPC;
+  const int g = threadIdx.x / Z::TPC;
+  const long long row = (long long)blockIdx.x * a.G + g;
+  const bool active = row < a.nrows;
+  const long long off = (active ? row : 0) * a.kzs;
+  const LdsRow idx{g * Z::RS * FUSED_B};
+  typename Z::Raw r0[3], r1[3];
+  cx<T> vel[3][E], om[3][E];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) Z::load(a.ws[b] + off, active, a.lim, t, r0[b]);
+  if (PREFETCH) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
+  }
+  Z::template c2r<3>(r0, lds, idx, t, twM, twN, scale, vel);  // u, v, w
+  if (!PREFETCH) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
+  }
+  if (PREFETCH && a.S > 0) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + b] + off, active, a.lim, t, r0[b]);
+  }
+  Z::template c2r<3>(r1, lds, idx, t, twM, twN, scale, om);  // omega
+  // u x omega, in place of omega
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const cx<T> u = vel[0][e], v = vel[1][e], w = vel[2][e];
+    const cx<T> ox = om[0][e], oy = om[1][e], oz = om[2][e];
+    om[0][e] = mk<T>(v.x * oz.x - w.x * oy.x, v.y * oz.y - w.y * oy.y);
+    om[1][e] = mk<T>(w.x * ox.x - u.x * oz.x, w.y * ox.y - u.y * oz.y);
+    om[2][e] = mk<T>(u.x * oy.x - v.x * ox.x, u.y * oy.y - v.y * ox.y);
+  }
+  {
+    cx<T>* dst[3] = {a.nl[0] + off, a.nl[1] + off, a.nl[2] + off};
+    Z::template r2c<3>(om, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+  }
+  for (int s = 0; s < a.S; ++s) {
+    if (!PREFETCH || s > 0) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s + b] + off, active, a.lim, t, r0[b]);
+    }
+    Z::template c2r<3>(r0, lds, idx, t, twM, twN, scale, om);  // grad phi
+    cx<T> n[1][E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const cx<T> u = vel[0][e], v = vel[1][e], w = vel[2][e];
+      n[0][e] = mk<T>(u.x * om[0][e].x + v.x * om[1][e].x + w.x * om[2][e].x,
+                      u.y * om[0][e].y + v.y * om[1][e].y + w.y * om[2][e].y);
+    }
+    cx<T>* dst[1] = {a.nl[3 + s] + off};
+    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+  }
 }
 
 }  // namespace gnss

@@ -108,6 +108,8 @@ SIGNATURES = {
     "gnss_timing_enable": (_I, [_P, _I]),
     "gnss_timing_report": (_I, [_P, ctypes.c_char_p, _I]),
     "gnss_bench_fft": (_I, [_P, _I, _D, _D]),
+    "gnss_autotune": (_I, [_P, _I, ctypes.c_char_p, _I]),
+    "gnss_set_tuning": (_I, [_P, ctypes.c_char_p]),
     "gnss_ensemble_create": (_I, [_I, ctypes.POINTER(Params), ctypes.POINTER(_P)]),
     "gnss_ensemble_destroy": (_I, [_P]),
     "gnss_ensemble_last_error": (ctypes.c_char_p, [_P]),
@@ -298,6 +300,15 @@ class Solver:
         self._check(self.lib.gnss_timing_report(self.ctx, buf, len(buf)))
         return json.loads(buf.value.decode() or "{}")
 
+    def autotune(self, reps: int = 5) -> dict:
+        """Time the compiled pass variants on this context and keep the fastest."""
+        buf = ctypes.create_string_buffer(1 << 14)
+        self._check(self.lib.gnss_autotune(self.ctx, int(reps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def set_tuning(self, spec: str):
+        self._check(self.lib.gnss_set_tuning(self.ctx, spec.encode()))
+
     def bench_fft(self, reps: int = 10):
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.gnss_bench_fft(self.ctx, int(reps), ctypes.byref(a), ctypes.byref(b)))
@@ -346,6 +357,13 @@ class Ensemble:
         self._check(self.lib.gnss_ensemble_stats(self.ens, int(i), ctypes.byref(o), None, None))
         return o.as_dict(S)
 
+    def stats_full(self, i: int):
+        c = self.cfgs[i]
+        S = len(c.sc)
+        o = StatsOut()
+        self._check(self.lib.gnss_ensemble_stats(self.ens, int(i), ctypes.byref(o), None, None))
+        return o.as_dict(S)
+
     def get_spectral(self, i: int, f: int) -> np.ndarray:
         c = self.cfgs[i]
         nx, ny, nz = c.n
@@ -372,3 +390,30 @@ def config(name: str, **over) -> Config:
     d.update(table[name])
     d.update(over)
     return Config(**d)
+
+
+def _u01(seed: int, k: int) -> float:
+    """Portable uniform draw in [0, 1) from a SplitMix64 hash of (seed, k)."""
+    m = 0xFFFFFFFFFFFFFFFF
+    z = (seed * 0x100000001B3 + k + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return (z >> 11) / float(1 << 53)
+
+
+def ensemble_configs(seed: int = 20261202, counts=((64, 32), (96, 16), (128, 8)),
+                     precision: int = FP32, dt: float = 1e-3) -> list:
+    """Uneven batched ensemble: `count` realisations per grid size n, member i with seed + i,
+    theta ~ U[0.03, 0.08] and Re ~ U[500, 2000] drawn from that seed, one scalar Sc = 0.7,
+    noise rms 0.05 on |m| <= n/8."""
+    out = []
+    i = 0
+    for n, count in counts:
+        for _ in range(count):
+            s = seed + i
+            out.append(Config(n=(n, n, n), re=500.0 + 1500.0 * _u01(s, 2), sc=(0.7,),
+                              theta=0.03 + 0.05 * _u01(s, 1), noise_amp=0.05, noise_kmax=n / 8.0,
+                              seed=s, dt=dt, precision=precision))
+            i += 1
+    return out

@@ -159,6 +159,16 @@ int gnss_timing_report(gnss_ctx* ctx, char* buf, int len);
  *   (engine only).  ms_fwd/ms_inv: average milliseconds per 3D R2C / C2R. */
 int gnss_bench_fft(gnss_ctx* ctx, int reps, double* ms_fwd, double* ms_inv);
 
+/* ≈ the plan selection of plan1dFFT / plan2dFFT (fftfuncs.cu:133-231), which the reference leaves
+ *   to cuFFT's planner.  The engine times its own compiled variants of the step's passes (column
+ *   tile width of the strided x/y passes, radix plan of the fused z pass) `reps` times each on this
+ *   context's buffers, keeps the fastest and writes a JSON report (candidates and choice) to
+ *   `report` (may be NULL).  reps == 0 only reports the current choice.  gnss_create applies the
+ *   choices stored in $GNSS_TUNING or tuning/gnss_tuning.json beside the library. */
+int gnss_autotune(gnss_ctx* ctx, int reps, char* report, int len);
+/* Forces a variant, e.g. "cx=8 cy=16 zf=1" (tests compare variants against each other). */
+int gnss_set_tuning(gnss_ctx* ctx, const char* spec);
+
 /* Batched ensemble of independent realisations of uneven sizes (no reference counterpart:
  *   the reference runs one realisation per process).  Members share one stream; one RK3 step
  *   of the whole ensemble is one hipGraph launch. */

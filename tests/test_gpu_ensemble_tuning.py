@@ -1,0 +1,74 @@
+"""Batched ensemble (one hipGraph per RK3 step) and the autotuned pass variants, on the GPU.
+
+Each ensemble member must equal the same realisation advanced alone by gnss_step (same kernels),
+and every tunable variant must give the same numbers as the default variant."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_ensemble_members_match_single_runs(g, engine, oracle):
+    cfgs = g.ensemble_configs(counts=((32, 3), (48, 2), (64, 2)), precision=g.FP64)
+    print(f"\n[ensemble {len(cfgs)} members]", flush=True)
+    ens = g.Ensemble(cfgs, engine)
+    ens.init()
+    ens.step(4)
+    for i, c in enumerate(cfgs):
+        s = g.Solver(c, engine).setup()
+        s.step(4)
+        for f in range(4):
+            assert rel(ens.get_spectral(i, f), s.get_spectral(f)) < 1e-14, (i, f)
+        st = ens.stats(i)
+        assert st["step"] == 4 and abs(st["t"] - 4 * c.dt) < 1e-15
+    # and one member against the oracle
+    o = g.Solver(cfgs[5], oracle).setup()
+    o.step(4)
+    for f in range(4):
+        assert rel(ens.get_spectral(5, f), o.get_spectral(f)) < 1e-12
+
+
+def test_ensemble_fp32_uneven(g, engine):
+    cfgs = g.ensemble_configs(counts=((64, 4), (96, 2), (128, 1)))
+    ens = g.Ensemble(cfgs, engine)
+    ens.init()
+    ens.step(3)
+    for i in (0, 4, 6):
+        s = g.Solver(cfgs[i], engine).setup()
+        s.step(3)
+        assert rel(ens.get_spectral(i, 0), s.get_spectral(0)) < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tuning_variants_agree(g, engine, prec):
+    p = g.FP64 if prec == "fp64" else g.FP32
+    cfg = g.Config(n=(96, 64, 128), re=500.0, sc=(0.7,), theta=0.08, noise_kmax=8.0, noise_amp=0.2, dt=1e-3,
+                   precision=p)
+    specs = ["cx=4 cy=4 zf=0", "cx=8 cy=8 zf=0", "cx=4 cy=8 zf=0"] if prec == "fp64" else \
+        ["cx=8 cy=8 zf=0", "cx=16 cy=16 zf=0", "cx=16 cy=8 zf=1", "cx=8 cy=16 zf=1"]
+    ref = None
+    for spec in specs:
+        s = g.Solver(cfg, engine).setup()
+        s.set_tuning(spec)
+        s.step(2)
+        out = [s.get_spectral(f) for f in range(4)]
+        if ref is None:
+            ref = out
+        else:
+            tol = 1e-13 if prec == "fp64" else 2e-6
+            for f in range(4):
+                assert rel(out[f], ref[f]) < tol, (spec, f)
+
+
+def test_autotune_reports_and_applies(g, engine):
+    s = g.Solver(g.Config(n=(64, 64, 64), precision=g.FP32), engine).setup()
+    rep = s.autotune(3)
+    assert "choice" in rep and "candidates" in rep
+    assert any(k.startswith("cx=") for k in rep["candidates"])
+    assert s.autotune(0)["choice"] == rep["choice"]
+    with pytest.raises(g.GnssError):
+        s.set_tuning("cx=3")

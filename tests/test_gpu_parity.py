@@ -107,3 +107,34 @@ def test_stats_fp64(g, engine, oracle):
     assert abs(sa["divergence"]) < 1e-13
     assert np.abs(ya - yb).max() <= 1e-12 * np.abs(yb).max()
     assert np.abs(pa - pb).max() <= 1e-12 * np.abs(pb).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_matches_unfused(g, engine, prec):
+    """The fused z kernel (C2R + products + R2C in registers) equals the separate-pass step."""
+    p = g.FP64 if prec == "fp64" else g.FP32
+    base = dict(n=(64, 48, 96), re=400.0, sc=(0.7, 2.0), theta=0.08, noise_kmax=8.0, noise_amp=0.2, dt=1e-3,
+                precision=p)
+    a = g.Solver(g.Config(**base), engine).setup()
+    b = g.Solver(g.Config(**base, flags=g.FLAG_UNFUSED), engine).setup()
+    a.step(3)
+    b.step(3)
+    tol = 1e-13 if prec == "fp64" else 1e-5
+    for f in range(5):
+        assert rel(a.get_spectral(f), b.get_spectral(f)) < tol, f
+
+
+def test_fp32_error_vs_fp64(g, engine):
+    """FP32 form against the FP64 plain form on the same seed (64^3 jet, 100 steps)."""
+    a = g.Solver(g.config("jet64", precision=g.FP32), engine).setup()
+    b = g.Solver(g.config("jet64"), engine).setup()
+    a.step(100)
+    b.step(100)
+    for f in range(4):
+        assert rel(a.get_spectral(f), b.get_spectral(f)) < 1e-5, f
+    sa, ya, _ = a.stats()
+    sb, yb, _ = b.stats()
+    for k in ["ke", "epsilon", "cfl"]:
+        assert abs(sa[k] - sb[k]) < 1e-4 * abs(sb[k]), k
+    assert abs(sa["chi"][0] - sb["chi"][0]) < 1e-4 * sb["chi"][0]
+    assert sa["divergence"] < 1e-6
