@@ -13,6 +13,7 @@
 #include "engine.h"
 #include "host_util.h"
 #include "kernels_fft.h"
+#include "kernels_mfma.h"
 #include "kernels_step.h"
 
 namespace gnss {
@@ -663,6 +664,27 @@ class Engine : public EngineBase {
     }
   }
 
+  // First radix-16 stage on MFMA (split FP16, FP32 accumulate); FP32 lengths 256, 512, 1024.
+  bool use_mfma(int L) const {
+    return sizeof(T) == 4 && (p_.flags & GNSS_FLAG_MFMA_R16) && (L == 256 || L == 512 || L == 1024);
+  }
+  template <int L, int SIGN, int CC>
+  void mfma_launch(C* data, const ColArgs& a) {
+    if constexpr (sizeof(T) == 4) {
+      const size_t lds = (size_t)L * CC * sizeof(C);
+      const long long blocks = (long long)a.nchunks * a.nouter;
+      hipLaunchKernelGGL((k_col_mfma<L, SIGN, CC>), dim3((unsigned)blocks), dim3(CC * (L / 16)), lds, s_,
+                         data, tw_.at(L), a);
+    }
+  }
+  template <int SIGN>
+  void mfma_dispatch(int L, C* data, const ColArgs& a, int CC) {
+    if (L == 1024) return mfma_launch<1024, SIGN, 8>(data, a);
+    if (L == 512) return CC == 16 ? mfma_launch<512, SIGN, 16>(data, a) : mfma_launch<512, SIGN, 8>(data, a);
+    if (L == 256) return CC == 16 ? mfma_launch<256, SIGN, 16>(data, a) : mfma_launch<256, SIGN, 8>(data, a);
+    throw Error(GNSS_ERR_UNSUPPORTED, "MFMA radix-16 stage needs length 256, 512 or 1024");
+  }
+
   template <int SIGN>
   void col_dispatch_l(int L, C* data, const ColArgs& a, int CC, int G) {
     switch (L) {
@@ -730,6 +752,16 @@ class Engine : public EngineBase {
     auto rows = [&](int keep) { return keep < 0 ? L : std::min(L, 2 * keep + 1); };
     const double bytes = (double)a.nouter * a.nchunks * CC * (rows(sp.in_keep) + rows(sp.out_keep)) * sizeof(C);
     const char* name = sp.axis == 0 ? (sp.sign < 0 ? "fft_x_fwd" : "fft_x_inv") : (sp.sign < 0 ? "fft_y_fwd" : "fft_y_inv");
+    if (use_mfma(L)) {
+      a.G = 1;
+      const char* mname = sp.axis == 0 ? (sp.sign < 0 ? "fft_x_fwd_mfma" : "fft_x_inv_mfma")
+                                       : (sp.sign < 0 ? "fft_y_fwd_mfma" : "fft_y_inv_mfma");
+      timed(mname, bytes, [&] {
+        if (sp.sign < 0) mfma_dispatch<-1>(L, data, a, CC);
+        else mfma_dispatch<+1>(L, data, a, CC);
+      });
+      return;
+    }
     timed(name, bytes, [&] {
       if (sp.sign < 0) col_dispatch_l<-1>(L, data, a, CC, G);
       else col_dispatch_l<+1>(L, data, a, CC, G);

@@ -138,3 +138,32 @@ def test_fp32_error_vs_fp64(g, engine):
         assert abs(sa[k] - sb[k]) < 1e-4 * abs(sb[k]), k
     assert abs(sa["chi"][0] - sb["chi"][0]) < 1e-4 * sb["chi"][0]
     assert sa["divergence"] < 1e-6
+
+
+@pytest.mark.parametrize("n", [(256, 512, 16), (1024, 256, 16)], ids=lambda n: "x".join(map(str, n)))
+def test_mfma_radix16_transform(g, engine, n):
+    """FP32 transform with the split-FP16 MFMA radix-16 stage vs numpy and vs the VALU path."""
+    print(f"\n[mfma fft {n}]", flush=True)
+    x = np.random.default_rng(11).standard_normal(n)
+    ref = np.fft.rfftn(x).transpose(1, 0, 2)
+    out = {}
+    for flags in (0, g.FLAG_MFMA_R16):
+        a = g.Solver(g.Config(n=n, sc=(), precision=g.FP32, flags=flags), engine)
+        a.set_physical(0, x)
+        a.forward(0)
+        out[flags] = a.get_spectral(0)
+        a.inverse(0)
+        assert rel(a.get_physical(0), x) < 3e-6
+    assert rel(out[g.FLAG_MFMA_R16], ref) < 3e-6
+    assert rel(out[g.FLAG_MFMA_R16], out[0]) < 3e-6
+
+
+def test_mfma_step_matches_valu(g, engine):
+    base = dict(n=(256, 256, 32), re=800.0, sc=(1.0,), theta=0.05, noise_kmax=12.0, dt=5e-4,
+                precision=g.FP32)
+    a = g.Solver(g.Config(**base), engine).setup()
+    b = g.Solver(g.Config(**base, flags=g.FLAG_MFMA_R16), engine).setup()
+    a.step(5)
+    b.step(5)
+    for f in range(4):
+        assert rel(b.get_spectral(f), a.get_spectral(f)) < 1e-5
