@@ -251,3 +251,85 @@ GNSS_API int gnss_set_tuning(gnss_ctx* c, const char* spec) {
   if (!spec) return GNSS_ERR_ARG;
   return guard(c, [&] { c->eng->set_tuning(spec); });
 }
+
+// ---------------------------------------------------------------------------------------------
+// Raw field files in the reference's format (iofuncs.cu:329-358 writer, 462-497 reader).
+// ---------------------------------------------------------------------------------------------
+static std::string field_name(int f) {
+  static const char* vel[3] = {"u", "v", "w"};
+  if (f < 3) return vel[f];
+  return f == 3 ? std::string("z") : "z" + std::to_string(f - 3);
+}
+
+static void write_raw(const std::string& path, const double* a, long long n) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) throw Error(GNSS_ERR_ARG, "cannot open " + path + " for writing");
+  const double hdr = (double)n * (double)sizeof(double);
+  const bool ok = fwrite(&hdr, sizeof(double), 1, f) == 1 && (long long)fwrite(a, sizeof(double), n, f) == n;
+  fclose(f);
+  if (!ok) throw Error(GNSS_ERR_ARG, "short write to " + path);
+}
+
+static void read_raw(const std::string& path, double* a, long long n) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) throw Error(GNSS_ERR_ARG, "cannot open " + path);
+  double hdr = 0;
+  const bool got = fread(&hdr, sizeof(double), 1, f) == 1;
+  if (!got || hdr != (double)n * (double)sizeof(double)) {
+    fclose(f);
+    throw Error(GNSS_ERR_ARG, path + ": size header does not match nx*ny*nz doubles");
+  }
+  const bool ok = (long long)fread(a, sizeof(double), n, f) == n;
+  fclose(f);
+  if (!ok) throw Error(GNSS_ERR_ARG, "short read from " + path);
+}
+
+GNSS_API int gnss_save_fields(gnss_ctx* c, const char* dir, int iter) {
+  if (!dir) return GNSS_ERR_ARG;
+  return guard(c, [&] {
+    gnss_sizes sz;
+    c->eng->sizes(&sz);
+    std::vector<double> buf((size_t)sz.n_physical);
+    for (int f = 0; f < sz.n_fields; ++f) {
+      c->eng->inverse(f);
+      c->eng->get_field(f, GNSS_PHYSICAL, buf.data());
+      write_raw(std::string(dir) + "/" + field_name(f) + "." + std::to_string(iter), buf.data(), sz.n_physical);
+    }
+    double t;
+    int64_t st;
+    c->eng->time(&t, &st);
+    const std::string sp = std::string(dir) + "/gnss_state." + std::to_string(iter);
+    FILE* fp = fopen(sp.c_str(), "w");
+    if (!fp) throw Error(GNSS_ERR_ARG, "cannot open " + sp);
+    fprintf(fp, "step %lld t %.17g\n", (long long)st, t);
+    fclose(fp);
+  });
+}
+
+GNSS_API int gnss_load_fields(gnss_ctx* c, const char* dir, int iter) {
+  if (!dir) return GNSS_ERR_ARG;
+  return guard(c, [&] {
+    gnss_sizes sz;
+    c->eng->sizes(&sz);
+    std::vector<double> buf((size_t)sz.n_physical);
+    for (int f = 0; f < sz.n_fields; ++f) {
+      read_raw(std::string(dir) + "/" + field_name(f) + "." + std::to_string(iter), buf.data(), sz.n_physical);
+      c->eng->set_field(f, GNSS_PHYSICAL, buf.data());
+      c->eng->forward(f);
+    }
+    c->eng->dealias();
+    long long st = iter;
+    double t = (double)iter * c->p.dt;
+    const std::string sp = std::string(dir) + "/gnss_state." + std::to_string(iter);
+    if (FILE* fp = fopen(sp.c_str(), "r")) {
+      long long a;
+      double b;
+      if (fscanf(fp, "step %lld t %lf", &a, &b) == 2) {
+        st = a;
+        t = b;
+      }
+      fclose(fp);
+    }
+    c->eng->set_clock(st, t);
+  });
+}

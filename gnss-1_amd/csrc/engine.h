@@ -69,6 +69,7 @@ class EngineBase {
   virtual void set_field(int field, int space, const double* host) = 0;
   virtual void sizes(gnss_sizes* out) const = 0;
   virtual void time(double* t, int64_t* step) const = 0;
+  virtual void set_clock(int64_t step, double t) = 0;
   virtual void timing_enable(bool on) = 0;
   virtual std::string timing_report() = 0;
   virtual void bench_fft(int reps, double* ms_fwd, double* ms_inv) = 0;

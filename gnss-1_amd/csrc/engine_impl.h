@@ -76,6 +76,10 @@ class Engine : public EngineBase {
     if (t) *t = t_;
     if (st) *st = step_;
   }
+  void set_clock(int64_t st, double t) override {
+    step_ = st;
+    t_ = t;
+  }
 
   void init_wavenumbers() override {
     const double pi = 3.14159265358979323846;

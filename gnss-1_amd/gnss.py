@@ -105,6 +105,8 @@ SIGNATURES = {
     "gnss_get_field": (_I, [_P, _I, _I, _D]),
     "gnss_set_field": (_I, [_P, _I, _I, _D]),
     "gnss_get_time": (_I, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
+    "gnss_save_fields": (_I, [_P, ctypes.c_char_p, _I]),
+    "gnss_load_fields": (_I, [_P, ctypes.c_char_p, _I]),
     "gnss_timing_enable": (_I, [_P, _I]),
     "gnss_timing_report": (_I, [_P, ctypes.c_char_p, _I]),
     "gnss_bench_fft": (_I, [_P, _I, _D, _D]),
@@ -251,6 +253,13 @@ class Solver:
                                         _dptr(yp) if yp is not None else None,
                                         _dptr(sp) if sp is not None else None))
         return o.as_dict(self.S), yp, sp
+
+    def save_fields(self, directory: str, it: int):
+        """Checkpoint in the reference's raw physical-space format (<dir>/<u|v|w|z..>.<it>)."""
+        self._check(self.lib.gnss_save_fields(self.ctx, directory.encode(), int(it)))
+
+    def load_fields(self, directory: str, it: int):
+        self._check(self.lib.gnss_load_fields(self.ctx, directory.encode(), int(it)))
 
     def time(self):
         t, s = ctypes.c_double(), ctypes.c_int64()

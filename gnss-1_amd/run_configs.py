@@ -1,0 +1,132 @@
+"""Runs the engine on the measured workloads beyond the headline bench line and prints JSON.
+
+  jet256    FP64, 256^3, two scalars (Sc 0.7, 2.0), 200 steps, statistics every 20 steps
+  ensemble  FP32 batched ensemble 32 x 64^3 + 16 x 96^3 + 8 x 128^3, one hipGraph per RK3 step,
+            200 steps; per-member error against the FP64 form of sampled members
+  jet1024   FP32 at 1024^3 (50 steps) with the VALU radix stages and with the split-FP16 MFMA
+            radix-16 stage; errors against each other and against FP64 at 512^3 (same physical
+            parameters) compared on the 512^3 modes
+
+usage: python gnss-1_amd/run_configs.py {jet256,ensemble,jet1024} [--steps K]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import gnss  # noqa: E402
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_jet256(steps):
+    cfg = gnss.config("jet256")
+    s = gnss.Solver(cfg).setup()
+    out = {"workload": "jet256 fp64", "stats": []}
+    st, yp, sp = s.stats()
+    out["stats"].append(st)
+    t_step = 0.0
+    for c in range(20, steps + 1, 20):
+        t0 = time.perf_counter()
+        s.step(20)
+        t_step += time.perf_counter() - t0
+        st, yp, sp = s.stats()
+        out["stats"].append(st)
+        log(f"[jet256] step {st['step']} KE {st['ke']:.8e} eps {st['epsilon']:.6e} CFL {st['cfl']:.4f}")
+    out["steps_per_s"] = steps / t_step
+    out["grid_point_updates_per_s"] = steps / t_step * 256 ** 3
+    out["final_profiles_rows"] = int(yp.shape[0])
+    out["final_spectrum_peak_shell"] = int(np.argmax(sp[0]))
+    return out
+
+
+def run_ensemble(steps):
+    cfgs = gnss.ensemble_configs()
+    t0 = time.perf_counter()
+    ens = gnss.Ensemble(cfgs)
+    ens.init()
+    t_init = time.perf_counter() - t0
+    ens.step(2)  # warm-up replays
+    t0 = time.perf_counter()
+    ens.step(steps)
+    dt = time.perf_counter() - t0
+    pts = sum(c.n[0] * c.n[1] * c.n[2] for c in cfgs)
+    out = {"workload": "ensemble 32x64^3 + 16x96^3 + 8x128^3 fp32", "members": len(cfgs),
+           "init_s": t_init, "steps": steps, "ensemble_steps_per_s": steps / dt,
+           "member_steps_per_s": len(cfgs) * steps / dt, "grid_point_updates_per_s": pts * steps / dt}
+    # errors of sampled members against their FP64 form (same seed, same number of steps)
+    errs = {}
+    for i in (0, 31, 32, 47, 48, 55):
+        c64 = gnss.Config(**{**cfgs[i].__dict__, "precision": gnss.FP64})
+        s = gnss.Solver(c64).setup()
+        s.step(steps + 2)
+        e = [rel(ens.get_spectral(i, f), s.get_spectral(f)) for f in range(4)]
+        st32, st64 = ens.stats(i), s.stats(False, False)[0]
+        errs[i] = {"n": cfgs[i].n[0], "re": cfgs[i].re, "theta": cfgs[i].theta, "rel_l2_max": max(e),
+                   "ke_rel": abs(st32["ke"] - st64["ke"]) / st64["ke"]}
+        s.close()
+    out["fp32_vs_fp64"] = errs
+    return out
+
+
+def downsample(a, n_small):
+    """Spectral field [ky][kx][kz] of a large grid -> the modes of an n_small grid."""
+    ny, nx, nzh = a.shape
+    h = n_small // 2
+    iy = np.r_[0:h, ny - h:ny]
+    ix = np.r_[0:h, nx - h:nx]
+    return a[np.ix_(iy, ix, np.arange(h + 1))]
+
+
+def run_jet1024(steps):
+    out = {"workload": "jet1024 fp32 (VALU and MFMA split-FP16 radix-16)", "steps": steps}
+    fields = {}
+    for name, flags in (("mfma", gnss.FLAG_MFMA_R16), ("valu", 0)):
+        cfg = gnss.config("jet1024", precision=gnss.FP32, flags=flags)
+        log(f"[jet1024] {name}: init")
+        s = gnss.Solver(cfg).setup()
+        s.step(1)
+        t0 = time.perf_counter()
+        s.step(steps - 1)
+        dt = time.perf_counter() - t0
+        out[f"{name}_steps_per_s"] = (steps - 1) / dt
+        out[f"{name}_ms_per_step"] = 1000 * dt / (steps - 1)
+        st = s.stats(False, False)[0]
+        out[f"{name}_stats"] = {k: st[k] for k in ("ke", "epsilon", "cfl", "divergence")}
+        fields[name] = [downsample(s.get_spectral(f), 512) for f in range(4)]
+        log(f"[jet1024] {name}: {out[f'{name}_ms_per_step']:.1f} ms/step")
+        s.close()
+    out["mfma_vs_valu_rel_l2"] = [rel(fields["mfma"][f], fields["valu"][f]) for f in range(4)]
+    cfg = gnss.config("jet1024", n=(512,) * 3, precision=gnss.FP64)
+    s = gnss.Solver(cfg).setup()
+    s.step(steps)
+    ref = [s.get_spectral(f) * 8.0 for f in range(4)]  # unnormalised: 1024^3 / 512^3 = 8
+    mask = s.mask().astype(bool)
+    out["fp32_1024_vs_fp64_512_rel_l2"] = {
+        k: [rel(fields[k][f][mask], ref[f][mask]) for f in range(4)] for k in fields}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["jet256", "ensemble", "jet1024"])
+    ap.add_argument("--steps", type=int, default=0)
+    a = ap.parse_args()
+    steps = a.steps or {"jet256": 200, "ensemble": 200, "jet1024": 50}[a.which]
+    fn = {"jet256": run_jet256, "ensemble": run_ensemble, "jet1024": run_jet1024}[a.which]
+    print(json.dumps(fn(steps)))
+
+
+if __name__ == "__main__":
+    main()

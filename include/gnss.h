@@ -151,6 +151,17 @@ int gnss_get_field(gnss_ctx* ctx, int field, int space, double* host);
 int gnss_set_field(gnss_ctx* ctx, int field, int space, const double* host);
 int gnss_get_time(const gnss_ctx* ctx, double* t, int64_t* step);
 
+/* ≈ save3Dfields / write3Dfields_mgpu (iofuncs.cu:329-432): writes every field in physical space
+ *   to <dir>/<name>.<iter> in the reference's raw format — an 8-byte header holding the byte count
+ *   as a double, then nx*ny*nz doubles in [x][y][z] order — with names u, v, w, z (scalar 0) and
+ *   z1, z2, ... for further scalars, plus <dir>/gnss_state.<iter> ("step <n> t <time>").  The
+ *   RK3 integrator is self-starting, so fields + clock are the whole checkpoint. */
+int gnss_save_fields(gnss_ctx* ctx, const char* dir, int iter);
+/* ≈ loadData / importVelocity / importScalar (iofuncs.cu:462-535): reads the same files (the size
+ *   header must equal nx*ny*nz*8, as iofuncs.cu:475-479 checks), forward-transforms and dealiases
+ *   them, and restores the clock from gnss_state.<iter> (else step = iter, t = iter * dt). */
+int gnss_load_fields(gnss_ctx* ctx, const char* dir, int iter);
+
 /* Per-kernel timing with HIP events on the context's stream (engine only; the oracle returns
  *   GNSS_ERR_UNSUPPORTED).  The report is a JSON object: kernel -> {launches, ms, bytes}. */
 int gnss_timing_enable(gnss_ctx* ctx, int enable);

@@ -1029,6 +1029,70 @@ EXPORT int gnss_get_time(const gnss_ctx* c, double* t, int64_t* step) {
   return GNSS_OK;
 }
 
+/* Raw field files in the reference's format (iofuncs.cu:329-358 writer, 462-497 reader). */
+static void field_name(int f, char* out, size_t n) {
+  static const char* vel[3] = {"u", "v", "w"};
+  if (f < 3) snprintf(out, n, "%s", vel[f]);
+  else if (f == 3) snprintf(out, n, "z");
+  else snprintf(out, n, "z%d", f - 3);
+}
+
+EXPORT int gnss_save_fields(gnss_ctx* c, const char* dir, int iter) {
+  if (!c || !dir) return GNSS_ERR_ARG;
+  char nm[16], path[4096];
+  for (int f = 0; f < c->F; ++f) {
+    c2r3d(c, c->q[f], c->phys[f]);
+    field_name(f, nm, sizeof nm);
+    snprintf(path, sizeof path, "%s/%s.%d", dir, nm, iter);
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return set_err(c, GNSS_ERR_ARG, "cannot open %s", path);
+    const double hdr = (double)c->nph * (double)sizeof(double);
+    int ok = fwrite(&hdr, sizeof(double), 1, fp) == 1 && fwrite(c->phys[f], sizeof(double), c->nph, fp) == c->nph;
+    fclose(fp);
+    if (!ok) return set_err(c, GNSS_ERR_ARG, "short write to %s", path);
+  }
+  snprintf(path, sizeof path, "%s/gnss_state.%d", dir, iter);
+  FILE* fp = fopen(path, "w");
+  if (!fp) return set_err(c, GNSS_ERR_ARG, "cannot open %s", path);
+  fprintf(fp, "step %lld t %.17g\n", (long long)c->step, c->t);
+  fclose(fp);
+  return GNSS_OK;
+}
+
+EXPORT int gnss_load_fields(gnss_ctx* c, const char* dir, int iter) {
+  if (!c || !dir) return GNSS_ERR_ARG;
+  if (!c->have_k) return set_err(c, GNSS_ERR_STATE, "call gnss_init_wavenumbers first");
+  char nm[16], path[4096];
+  for (int f = 0; f < c->F; ++f) {
+    field_name(f, nm, sizeof nm);
+    snprintf(path, sizeof path, "%s/%s.%d", dir, nm, iter);
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return set_err(c, GNSS_ERR_ARG, "cannot open %s", path);
+    double hdr = 0;
+    if (fread(&hdr, sizeof(double), 1, fp) != 1 || hdr != (double)c->nph * (double)sizeof(double)) {
+      fclose(fp);
+      return set_err(c, GNSS_ERR_ARG, "%s: size header does not match nx*ny*nz doubles", path);
+    }
+    int ok = fread(c->phys[f], sizeof(double), c->nph, fp) == c->nph;
+    fclose(fp);
+    if (!ok) return set_err(c, GNSS_ERR_ARG, "short read from %s", path);
+    r2c3d(c, c->phys[f], c->q[f]);
+    apply_mask(c, c->q[f]);
+    memset(c->dq[f], 0, sizeof(cplx) * c->nsp);
+  }
+  c->step = iter;
+  c->t = (double)iter * c->p.dt;
+  snprintf(path, sizeof path, "%s/gnss_state.%d", dir, iter);
+  FILE* fp = fopen(path, "r");
+  if (fp) {
+    long long a;
+    double b;
+    if (fscanf(fp, "step %lld t %lf", &a, &b) == 2) { c->step = a; c->t = b; }
+    fclose(fp);
+  }
+  return GNSS_OK;
+}
+
 EXPORT int gnss_timing_enable(gnss_ctx* c, int e) { (void)e; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "no kernels on the CPU") : GNSS_ERR_ARG; }
 EXPORT int gnss_timing_report(gnss_ctx* c, char* b, int n) { (void)b; (void)n; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "no kernels on the CPU") : GNSS_ERR_ARG; }
 EXPORT int gnss_bench_fft(gnss_ctx* c, int r, double* a, double* b) { (void)r; (void)a; (void)b; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "engine only") : GNSS_ERR_ARG; }
