@@ -378,21 +378,23 @@ class Engine : public EngineBase {
   }
   std::string tuning_string() const {
     char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d", cx_, cy_, zf_);
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d", cx_, cy_, zf_, xf_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_;
     const char* p = spec.c_str();
     const char* q;
+    if ((q = strstr(p, "xf=")) || (q = strstr(p, "\"xf\": "))) xf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cx=")) || (q = strstr(p, "\"cx\": "))) cx = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cy=")) || (q = strstr(p, "\"cy\": "))) cy = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 1))
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 1) || xf < 0 || xf > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
     zf_ = zf;
+    xf_ = xf;
   }
 
   std::string autotune(int reps) override {
@@ -451,6 +453,43 @@ class Engine : public EngineBase {
         if (ms < best_ms) { best_ms = ms; best = v; }
       }
       zf_ = best;
+    }
+    // fused x passes (prologue / update inside the x transforms) or separate kernels: timed over
+    // whole middle substages on a snapshot of the state, restored afterwards
+    {
+      std::vector<void*> snap;
+      auto grab = [&](size_t bytes) {
+        void* v = nullptr;
+        GNSS_HIP(hipMalloc(&v, bytes));
+        snap.push_back(v);
+        return v;
+      };
+      try {
+        for (int f = 0; f < F_; ++f) {
+          GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+          GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), dq_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+        }
+        int best = xf_;
+        double best_ms = 1e30;
+        for (int v = 0; v <= 1; ++v) {
+          xf_ = v;
+          const double ms = time_it([&] { substage(1); });
+          snprintf(buf, sizeof buf, ", \"xf=%d\": %.5f", v, ms);
+          rep += buf;
+          if (ms < best_ms) { best_ms = ms; best = v; }
+        }
+        xf_ = best;
+        for (int f = 0; f < F_; ++f) {
+          GNSS_HIP(hipMemcpyAsync(q_[f], snap[2 * f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+          GNSS_HIP(hipMemcpyAsync(dq_[f], snap[2 * f + 1], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+        }
+        sync();
+      } catch (...) {
+        (void)hipStreamSynchronize(s_);
+        for (void* v : snap) (void)hipFree(v);
+        throw;
+      }
+      for (void* v : snap) (void)hipFree(v);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -888,6 +927,66 @@ class Engine : public EngineBase {
                        (size_t)(G * row_lds), s_, a, tw_.at(M), tw_.at(NZ), scale);
   }
 
+  // x passes fused with the spectral ends of a substage, one launch per field group
+  // (velocity, then each scalar).
+  bool fused_x_ok() const { return xf_ && !(p_.flags & GNSS_FLAG_MFMA_R16); }
+
+  template <int L, int CC, bool VEL>
+  void xgrp_launch(bool pro, int G, int nchunks, int scalar, const RkCoef& rc) {
+    constexpr int TPC = L / Plan<L>::E;
+    const int nouter = 2 * g_.keep_y + 1;
+    const long long blocks = (long long)nchunks * cdiv(nouter, G);
+    const size_t lds = (size_t)G * L * CC * sizeof(C);
+    if (pro)
+      hipLaunchKernelGGL((k_xinv_prologue<T, L, CC, VEL>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_,
+                         g_, ptrs(q_), ptrs(ws_), tkx_, tky_, tkz_, tw_.at(L), scalar, nchunks, G);
+    else
+      hipLaunchKernelGGL((k_xfwd_rk<T, L, CC, VEL>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_, g_,
+                         ptrs(q_), ptrs(dq_), ptrs(nl_), tkx_, tky_, tkz_, tw_.at(L), rc, scalar, nchunks, G);
+  }
+  template <int L>
+  void xgrp_c(bool pro, int G, int nchunks, int grp, const RkCoef& rc) {
+    const bool vel = grp < 0;
+    if constexpr (sizeof(T) == 8) {
+      if (cx_ == 8) vel ? xgrp_launch<L, 8, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 8, false>(pro, G, nchunks, grp, rc);
+      else vel ? xgrp_launch<L, 4, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 4, false>(pro, G, nchunks, grp, rc);
+    } else {
+      if (cx_ == 16) vel ? xgrp_launch<L, 16, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 16, false>(pro, G, nchunks, grp, rc);
+      else vel ? xgrp_launch<L, 8, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 8, false>(pro, G, nchunks, grp, rc);
+    }
+  }
+  // grp < 0: velocity group; grp = s: scalar s.
+  void x_fused(bool pro, int grp, const RkCoef& rc) {
+    const int L = nx_, CC = cx_;
+    const int TPC = L / elems_per_thread(L);
+    int G = std::max(1, 256 / (CC * TPC));
+    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
+    const int nchunks = kz_lim_ / CC;
+    const int nouter = 2 * g_.keep_y + 1;
+    const double kept_rows = 2 * g_.keep_x + 1;
+    const int nin = grp < 0 ? 3 : 1, nout = grp < 0 ? 6 : 3;
+    const double per_col = pro ? (nin * kept_rows + nout * (double)L)
+                               : (nin * (double)L + nin * kept_rows * (2 + (rc.first ? 0 : 1) + (rc.last ? 0 : 1)));
+    const double bytes = (double)nouter * nchunks * CC * per_col * sizeof(C);
+    timed(pro ? "fft_x_inv_prologue" : "fft_x_fwd_rk", bytes, [&] {
+      switch (L) {
+        case 16: xgrp_c<16>(pro, G, nchunks, grp, rc); break;
+        case 24: xgrp_c<24>(pro, G, nchunks, grp, rc); break;
+        case 32: xgrp_c<32>(pro, G, nchunks, grp, rc); break;
+        case 48: xgrp_c<48>(pro, G, nchunks, grp, rc); break;
+        case 64: xgrp_c<64>(pro, G, nchunks, grp, rc); break;
+        case 96: xgrp_c<96>(pro, G, nchunks, grp, rc); break;
+        case 128: xgrp_c<128>(pro, G, nchunks, grp, rc); break;
+        case 192: xgrp_c<192>(pro, G, nchunks, grp, rc); break;
+        case 256: xgrp_c<256>(pro, G, nchunks, grp, rc); break;
+        case 384: xgrp_c<384>(pro, G, nchunks, grp, rc); break;
+        case 512: xgrp_c<512>(pro, G, nchunks, grp, rc); break;
+        case 1024: xgrp_c<1024>(pro, G, nchunks, grp, rc); break;
+        default: throw Error(GNSS_ERR_UNSUPPORTED, "fused x pass length " + std::to_string(L));
+      }
+    });
+  }
+
   // z pass of a substage fused with the physical products: ws (after the x, y inverse passes)
   // -> nl (before the y, x forward passes), kz < kz_lim_.
   void z_fused() {
@@ -903,33 +1002,7 @@ class Engine : public EngineBase {
     timed("fft_z_fused", bytes, [&] { GNSS_Z_SWITCH(zfused_launch, a, scale); });
   }
 
-  void substage(int s) {
-    const bool unfused = (p_.flags & GNSS_FLAG_UNFUSED) != 0;
-    const Box b = kept_box();
-    const long long nb = (long long)b.bx * b.by * b.bz;
-    timed("k_prologue", (double)nb * (F_ + W_) * sizeof(C), [&] {
-      hipLaunchKernelGGL(k_prologue<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_),
-                         ptrs(ws_), tkx_, tky_, tkz_, S_, nb);
-    });
-    if (unfused) {
-      ensure_unfused_buffers();
-      for (int w = 0; w < W_; ++w) inv3d(ws_[w], wp_[w], true);
-      timed("k_products", (double)nph_ * (W_ + F_) * sizeof(T), [&] {
-        hipLaunchKernelGGL(k_products<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, rptrs(wp_),
-                           rptrs(nlp_), S_, nph_);
-      });
-      for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
-    } else {
-      for (int w = 0; w < W_; ++w) {
-        col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
-        col_pass(ws_[w], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
-      }
-      z_fused();
-      for (int f = 0; f < F_; ++f) {
-        col_pass(nl_[f], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
-        col_pass(nl_[f], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
-      }
-    }
+  RkCoef rk_coef(int s) const {
     RkCoef c;
     c.a = RK_A[s];
     c.b = RK_B[s];
@@ -941,6 +1014,48 @@ class Engine : public EngineBase {
     for (int f = 0; f < MAXF; ++f) c.nu[f] = 0;
     for (int f = 0; f < 3; ++f) c.nu[f] = nu;
     for (int q = 0; q < S_; ++q) c.nu[3 + q] = nu / p_.sc[q];
+    return c;
+  }
+
+  void substage(int s) {
+    const bool unfused = (p_.flags & GNSS_FLAG_UNFUSED) != 0;
+    const Box b = kept_box();
+    const long long nb = (long long)b.bx * b.by * b.bz;
+    const RkCoef c = rk_coef(s);
+    const bool xf = !unfused && fused_x_ok();
+    if (xf) {
+      x_fused(true, -1, c);
+      for (int q = 0; q < S_; ++q) x_fused(true, q, c);
+    } else {
+      timed("k_prologue", (double)nb * (F_ + W_) * sizeof(C), [&] {
+        hipLaunchKernelGGL(k_prologue<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_),
+                           ptrs(ws_), tkx_, tky_, tkz_, S_, nb);
+      });
+    }
+    if (unfused) {
+      ensure_unfused_buffers();
+      for (int w = 0; w < W_; ++w) inv3d(ws_[w], wp_[w], true);
+      timed("k_products", (double)nph_ * (W_ + F_) * sizeof(T), [&] {
+        hipLaunchKernelGGL(k_products<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, rptrs(wp_),
+                           rptrs(nlp_), S_, nph_);
+      });
+      for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
+    } else {
+      for (int w = 0; w < W_; ++w) {
+        if (!xf) col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+        col_pass(ws_[w], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+      }
+      z_fused();
+      for (int f = 0; f < F_; ++f) {
+        col_pass(nl_[f], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
+        if (!xf) col_pass(nl_[f], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
+      }
+    }
+    if (xf) {
+      x_fused(false, -1, c);
+      for (int q = 0; q < S_; ++q) x_fused(false, q, c);
+      return;
+    }
     const double rk_bytes = (double)nb * F_ * sizeof(C) * (c.first ? 3 : (c.last ? 4 : 5));
     timed("k_rk", rk_bytes, [&] {
       hipLaunchKernelGGL(k_rk<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_), ptrs(dq_),
@@ -950,7 +1065,7 @@ class Engine : public EngineBase {
 
   // -------------------------------------------------------------------------------------------
   gnss_params p_;
-  int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0;
+  int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

@@ -1,6 +1,7 @@
 // kernels_step.h — pointwise step kernels and FP64 statistics reductions.
 #pragma once
 #include "fft_device.h"
+#include "kernels_fft.h"
 
 namespace gnss {
 
@@ -413,6 +414,168 @@ __global__ void k_finite(Geom g, CPtrs<T> q, int F, int* __restrict__ flag) {
   for (int f = 0; f < F; ++f) {
     const cx<T> v = q.p[f][i * g.kzs];
     if (!isfinite((double)v.x) || !isfinite((double)v.y)) flag[0] = 1;
+  }
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// x passes fused with the spectral ends of a substage.
+// ---------------------------------------------------------------------------------------------
+
+// Inverse x pass of the work fields of one field group, fed straight from the state.  Per
+// (ky kept, kz chunk) tile it loads the group's state columns once (kept kx rows only), forms
+// each work field in registers and transforms it:
+//   VEL:     u, v, w -> u, v, w, omega = i k x u   (calcOmega*Kernel_mgpu, solver.cu:68-114)
+//   scalar:  phi_s   -> i k phi_s                  (gradientKernel_mgpu, solver.cu:233-256)
+// Replaces k_prologue and the separate inverse x passes of those work fields.
+template <typename T, int L, int C, bool VEL>
+__global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
+k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, const T* __restrict__ kyv,
+                const T* __restrict__ kzv, const cx<T>* __restrict__ tw, int scalar, int nchunks, int G) {
+  constexpr int NIN = VEL ? 3 : 1;
+  constexpr int NOUT = VEL ? 6 : 3;
+  constexpr int E = Plan<L>::E;
+  constexpr int TPC = L / E;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = (tid / C) % TPC;
+  const int gg = tid / (C * TPC);
+  const int chunk = blockIdx.x % nchunks;
+  const int o = (blockIdx.x / nchunks) * G + gg;
+  const int nouter = 2 * g.keep_y + 1;
+  const bool active = o < nouter;
+  const int iy = outer_index(active ? o : 0, g.keep_y, g.ny);
+  const int iz = chunk * C + c;
+  const long long col = (long long)iy * g.kzs + iz;
+  const long long estride = (long long)g.ny * g.kzs;
+  const T ky = kyv[iy], kz = kzv[iz];
+  const int in0 = VEL ? 0 : 3 + scalar;
+  const int out0 = VEL ? 0 : 6 + 3 * scalar;
+  cx<T> qv[NIN][E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + e * TPC;
+    const bool ld = active && row_kept(p, L, g.keep_x);
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) qv[f][e] = ld ? q.p[in0 + f][col + p * estride] : mk<T>(0, 0);
+  }
+  const LdsCol idx{gg * L * C + c, C};
+#pragma unroll 1
+  for (int w = 0; w < NOUT; ++w) {
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const T kx = kxv[t + e * TPC];
+      if (VEL) {
+        const cx<T> u = qv[0][e], vv = qv[1][e], ww = qv[NIN - 1][e];
+        if (w < 3) v[e] = w == 0 ? u : (w == 1 ? vv : ww);
+        else if (w == 3) v[e] = csub(ik(ky, ww), ik(kz, vv));
+        else if (w == 4) v[e] = csub(ik(kz, u), ik(kx, ww));
+        else v[e] = csub(ik(kx, vv), ik(ky, u));
+      } else {
+        v[e] = ik(w == 0 ? kx : (w == 1 ? ky : kz), qv[0][e]);
+      }
+    }
+    __syncthreads();  // the previous field's last LDS reads are done
+    fft<T, L, +1>(v, lds, t, idx, tw);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) ws.p[out0 + w][col + (long long)(t + e * TPC) * estride] = v[e];
+    }
+  }
+}
+
+// Forward x pass of one field group's nonlinear terms fused with the projection (velocity),
+// the sign of the scalar term (computeRHSKernel_mgpu solver.cu:461-521), the mask and the RK3-IF
+// update of the state (replaces solver.cu:547-657): the transformed nonlinear terms never return
+// to HBM.
+template <typename T, int L, int C, bool VEL>
+__global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
+k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kxv,
+          const T* __restrict__ kyv, const T* __restrict__ kzv, const cx<T>* __restrict__ tw, RkCoef rc,
+          int scalar, int nchunks, int G) {
+  constexpr int NF = VEL ? 3 : 1;
+  constexpr int E = Plan<L>::E;
+  constexpr int TPC = L / E;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = (tid / C) % TPC;
+  const int gg = tid / (C * TPC);
+  const int chunk = blockIdx.x % nchunks;
+  const int o = (blockIdx.x / nchunks) * G + gg;
+  const int nouter = 2 * g.keep_y + 1;
+  const bool active = o < nouter;
+  const int iy = outer_index(active ? o : 0, g.keep_y, g.ny);
+  const int iz = chunk * C + c;
+  const long long col = (long long)iy * g.kzs + iz;
+  const long long estride = (long long)g.ny * g.kzs;
+  const LdsCol idx{gg * L * C + c, C};
+  const int f0 = VEL ? 0 : 3 + scalar;
+  cx<T> nv[NF][E];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      nv[f][e] = active ? nl.p[f0 + f][col + (long long)(t + e * TPC) * estride] : mk<T>(0, 0);
+    if (f > 0) __syncthreads();
+    fft<T, L, -1>(nv[f], lds, t, idx, tw);
+  }
+  if (!active) return;
+  const T ky = kyv[iy], kz = kzv[iz];
+  const int my = mode_of(iy, g.ny), mz = mode_of(iz, g.nz);
+  const T a = (T)rc.a, bb = (T)rc.b, dt = (T)rc.dt;
+  const T nu = (T)rc.nu[f0];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + e * TPC;
+    if (!row_kept(p, L, g.keep_x)) continue;
+    const long long o2 = col + (long long)p * estride;
+    if (iz >= g.nzh || !kept(g, mode_of(p, g.nx), my, mz)) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        q.p[f0 + f][o2] = mk<T>(0, 0);
+        if (!rc.last) dq.p[f0 + f][o2] = mk<T>(0, 0);
+      }
+      continue;
+    }
+    const T kx = kxv[p];
+    const T k2 = kx * kx + ky * ky + kz * kz;
+    cx<T> rhs[NF];
+    if (k2 == (T)0) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) rhs[f] = mk<T>(0, 0);
+    } else if (VEL) {
+      const cx<T> n0 = nv[0][e], n1 = nv[NF > 1 ? 1 : 0][e], n2 = nv[NF - 1][e];
+      cx<T> kn;
+      kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
+      kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
+      rhs[0] = csub(n0, cscale(kn, kx));
+      if (NF > 1) {
+        rhs[NF > 1 ? 1 : 0] = csub(n1, cscale(kn, ky));
+        rhs[NF - 1] = csub(n2, cscale(kn, kz));
+      }
+    } else {
+      rhs[0] = mk<T>(-nv[0][e].x, -nv[0][e].y);
+    }
+    const T ef = dexp<T>(-nu * k2 * (T)rc.tau);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      cx<T> d = cscale(rhs[f], dt);
+      if (!rc.first) {
+        const cx<T> old = dq.p[f0 + f][o2];
+        d.x = a * old.x + d.x;
+        d.y = a * old.y + d.y;
+      }
+      cx<T> qq = q.p[f0 + f][o2];
+      qq.x = ef * (qq.x + bb * d.x);
+      qq.y = ef * (qq.y + bb * d.y);
+      q.p[f0 + f][o2] = qq;
+      if (!rc.last) dq.p[f0 + f][o2] = cscale(d, ef);
+    }
   }
 }
 

@@ -172,12 +172,14 @@ int gnss_bench_fft(gnss_ctx* ctx, int reps, double* ms_fwd, double* ms_inv);
 
 /* ≈ the plan selection of plan1dFFT / plan2dFFT (fftfuncs.cu:133-231), which the reference leaves
  *   to cuFFT's planner.  The engine times its own compiled variants of the step's passes (column
- *   tile width of the strided x/y passes, radix plan of the fused z pass) `reps` times each on this
- *   context's buffers, keeps the fastest and writes a JSON report (candidates and choice) to
- *   `report` (may be NULL).  reps == 0 only reports the current choice.  gnss_create applies the
- *   choices stored in $GNSS_TUNING or tuning/gnss_tuning.json beside the library. */
+ *   tile width of the strided x/y passes, radix plan of the fused z pass, x passes fused with the
+ *   spectral prologue/update or not) `reps` times each on this context's buffers, keeps the
+ *   fastest and writes a JSON report (candidates and choice) to `report` (may be NULL).  The
+ *   state is snapshotted and restored around the timed substages.  reps == 0 only reports the
+ *   current choice.  gnss_create applies the choices stored in $GNSS_TUNING or
+ *   tuning/gnss_tuning.json beside the library. */
 int gnss_autotune(gnss_ctx* ctx, int reps, char* report, int len);
-/* Forces a variant, e.g. "cx=8 cy=16 zf=1" (tests compare variants against each other). */
+/* Forces a variant, e.g. "cx=8 cy=16 zf=1 xf=0" (tests compare variants against each other). */
 int gnss_set_tuning(gnss_ctx* ctx, const char* spec);
 
 /* Batched ensemble of independent realisations of uneven sizes (no reference counterpart:
