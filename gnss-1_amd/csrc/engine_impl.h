@@ -53,6 +53,11 @@ class Engine : public EngineBase {
     nph_ = (long long)nx_ * ny_ * nz_;
     load_tuning();
     GNSS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    {
+      int dev = 0;
+      GNSS_HIP(hipGetDevice(&dev));
+      GNSS_HIP(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     try {
       allocate();
     } catch (...) {
@@ -378,23 +383,25 @@ class Engine : public EngineBase {
   }
   std::string tuning_string() const {
     char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d", cx_, cy_, zf_, xf_);
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d", cx_, cy_, zf_, xf_, cp_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_;
     const char* p = spec.c_str();
     const char* q;
+    if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "xf=")) || (q = strstr(p, "\"xf\": "))) xf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cx=")) || (q = strstr(p, "\"cx\": "))) cx = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cy=")) || (q = strstr(p, "\"cy\": "))) cy = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 1) || xf < 0 || xf > 1)
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 1) || xf < 0 || xf > 1 || cp < 0 || cp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
     zf_ = zf;
     xf_ = xf;
+    cp_ = cp;
   }
 
   std::string autotune(int reps) override {
@@ -441,6 +448,24 @@ class Engine : public EngineBase {
         if (ms < best_ms) { best_ms = ms; best = c; }
       }
       (axis == 0 ? cx_ : cy_) = best;
+    }
+    {
+      // persistent pipelined column passes or one block per tile
+      int best = cp_;
+      double best_ms = 1e30;
+      for (int v = 0; v <= 1; ++v) {
+        cp_ = v;
+        const double ms = time_it([&] {
+          col_pass(ws_[0], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+          col_pass(ws_[0], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
+          col_pass(ws_[0], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+          col_pass(ws_[0], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
+        });
+        snprintf(buf, sizeof buf, ", \"cp=%d\": %.5f", v, ms);
+        rep += buf;
+        if (ms < best_ms) { best_ms = ms; best = v; }
+      }
+      cp_ = best;
     }
     {
       int best = zf_;
@@ -692,6 +717,15 @@ class Engine : public EngineBase {
     const int threads = CC * TPC * G;
     const size_t lds = (size_t)G * L * CC * sizeof(C);
     const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
+    if (cp_) {
+      // persistent: as many blocks as stay resident (LDS and wave limits of a CU), all CUs
+      const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds,
+                                                                           2048 / threads));
+      const long long grid = std::min<long long>(blocks, per_cu * n_cu_);
+      hipLaunchKernelGGL((k_col_pipe<T, L, SIGN, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, data,
+                         tw_.at(L), a, (int)blocks);
+      return;
+    }
     hipLaunchKernelGGL((k_col<T, L, SIGN, CC>), dim3((unsigned)blocks), dim3(threads), lds, s_, data,
                        tw_.at(L), a);
   }
@@ -1066,6 +1100,7 @@ class Engine : public EngineBase {
   // -------------------------------------------------------------------------------------------
   gnss_params p_;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
+  int cp_ = 0, n_cu_ = 256;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

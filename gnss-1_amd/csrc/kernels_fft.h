@@ -80,6 +80,61 @@ k_col(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a) {
   }
 }
 
+// Persistent, software-pipelined form of k_col: each block walks tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ... and issues the HBM loads of its next tile before transforming the
+// current one (stage barriers wait only for LDS traffic, so the loads stay in flight).
+template <typename T, int L, int SIGN, int C>
+__global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
+k_col_pipe(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a, int ntiles) {
+  constexpr int E = Plan<L>::E;
+  constexpr int TPC = L / E;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = (tid / C) % TPC;
+  const int g = tid / (C * TPC);
+  auto base_of = [&](int tile, bool& active) {
+    const int chunk = tile % a.nchunks;
+    const int o = (tile / a.nchunks) * a.G + g;
+    active = o < a.nouter;
+    const long long io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
+    return data + io * a.ostride + (long long)chunk * C + c;
+  };
+  auto load = [&](int tile, cx<T>* v) {
+    bool active;
+    const cx<T>* base = base_of(tile, active);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int p = t + e * TPC;
+      v[e] = (active && row_kept(p, L, a.in_keep)) ? base[(long long)p * a.estride] : mk<T>(0, 0);
+    }
+  };
+  int tile = blockIdx.x;
+  cx<T> v[E], nv[E];
+  load(tile, v);
+  const LdsCol idx{g * L * C + c, C};
+  while (true) {
+    const int next = tile + gridDim.x;
+    const bool more = next < ntiles;
+    if (more) load(next, nv);
+    fft<T, L, SIGN>(v, lds, t, idx, tw);
+    bool active;
+    cx<T>* base = base_of(tile, active);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int p = t + e * TPC;
+        if (row_kept(p, L, a.out_keep)) base[(long long)p * a.estride] = v[e];
+      }
+    }
+    if (!more) break;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = nv[e];
+    tile = next;
+  }
+}
+
 // Row stride (complex entries) of the LDS tile used by the z passes: M+1 values plus padding.
 __host__ __device__ constexpr int zrow_stride(int M) { return M + 2 + (M >> 4); }
 
