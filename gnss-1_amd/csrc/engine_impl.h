@@ -395,7 +395,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "cx=")) || (q = strstr(p, "\"cx\": "))) cx = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cy=")) || (q = strstr(p, "\"cy\": "))) cy = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 1) || xf < 0 || xf > 1 || cp < 0 || cp > 1)
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 2) || xf < 0 || xf > 1 || cp < 0 || cp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
@@ -470,7 +470,7 @@ class Engine : public EngineBase {
     {
       int best = zf_;
       double best_ms = 1e30;
-      for (int v = 0; v <= (sizeof(T) == 8 ? 0 : 1); ++v) {
+      for (int v = 0; v <= (sizeof(T) == 8 ? 0 : 2); ++v) {
         zf_ = v;
         const double ms = time_it([&] { z_fused(); });
         snprintf(buf, sizeof buf, ", \"zf=%d\": %.5f", v, ms);
@@ -943,6 +943,7 @@ class Engine : public EngineBase {
   void zfused_launch(const ZFusedArgs<T>& a0, T scale) {
     if constexpr (sizeof(T) == 4) {
       if (zf_ == 1) return zfused_launch_v<NZ, 1>(a0, scale);
+      if (zf_ == 2) return zfused_launch_v<NZ, 2>(a0, scale);
     }
     zfused_launch_v<NZ, 0>(a0, scale);
   }

@@ -301,10 +301,21 @@ k_c2r_z(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __restri
   for (int e = 0; e < Z::E; ++e) dst[t + e * Z::TPC] = v[0][e];
 }
 
-// Plans of the fused z kernel (chosen by the autotuner): V = 0 the fewest elements per thread
-// (bounded registers), V = 1 the default plan (fewer radix stages, more registers).
-template <typename T, int M, int V> struct FusedPlan { using type = PlanLow<M>; };
-template <typename T, int M> struct FusedPlan<T, M, 1> { using type = Plan<M>; };
+// Variants of the fused z kernel (chosen by the autotuner): V = 0 the fewest elements per thread
+// with the next rows' loads issued early (FP32), V = 1 the default plan (fewer radix stages, more
+// registers), V = 2 the fewest elements per thread without early loads (fewest registers).
+template <typename T, int M, int V> struct FusedPlan {
+  using type = PlanLow<M>;
+  static constexpr bool prefetch = sizeof(T) == 4;
+};
+template <typename T, int M> struct FusedPlan<T, M, 1> {
+  using type = Plan<M>;
+  static constexpr bool prefetch = sizeof(T) == 4;
+};
+template <typename T, int M> struct FusedPlan<T, M, 2> {
+  using type = PlanLow<M>;
+  static constexpr bool prefetch = false;
+};
 
 template <typename T> struct ZFusedArgs {
   const cx<T>* ws[18];   // u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar
@@ -340,7 +351,7 @@ __global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>:
 k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, T scale) {
   using Z = ZRow<T, NZ, typename FusedPlan<T, NZ / 2, V>::type>;
   constexpr int E = Z::E;
-  constexpr bool PREFETCH = sizeof(T) == 4;
+  constexpr bool PREFETCH = FusedPlan<T, NZ / 2, V>::prefetch;
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int t = threadIdx.x % Z::TPC;
