@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <sstream>
 
 #include "engine.h"
@@ -35,8 +36,12 @@ class Engine : public EngineBase {
     nx_ = p.nx; ny_ = p.ny; nz_ = p.nz;
     nzh_ = nz_ / 2 + 1;
     S_ = p.n_scalars;
+    colloid_ = (p.flags & GNSS_FLAG_COLLOID) ? 1 : 0;
+    if (colloid_ && S_ < 2)
+      throw Error(GNSS_ERR_ARG, "GNSS_FLAG_COLLOID needs n_scalars >= 2 (scalar 0 carries the colloid)");
+    if (colloid_ && !std::isfinite(p.alpha)) throw Error(GNSS_ERR_ARG, "alpha must be finite");
     F_ = 3 + S_;
-    W_ = 6 + 3 * S_;
+    W_ = 6 + 3 * S_ + 2 * colloid_;
     cx_ = col_chunk(nx_);
     cy_ = col_chunk(ny_);
     zf_ = 0;
@@ -378,7 +383,8 @@ class Engine : public EngineBase {
   // ------------------------------------------------------------------------------------------
   std::string tuning_key() const {
     char b[96];
-    snprintf(b, sizeof b, "%s %d %d %d %d", sizeof(T) == 8 ? "f64" : "f32", nx_, ny_, nz_, S_);
+    snprintf(b, sizeof b, "%s %d %d %d %d%s", sizeof(T) == 8 ? "f64" : "f32", nx_, ny_, nz_, S_,
+             colloid_ ? "c" : "");
     return b;
   }
   std::string tuning_string() const {
@@ -966,31 +972,36 @@ class Engine : public EngineBase {
   // (velocity, then each scalar).
   bool fused_x_ok() const { return xf_ && !(p_.flags & GNSS_FLAG_MFMA_R16); }
 
-  template <int L, int CC, bool VEL>
+  template <int L, int CC, int GRP>
   void xgrp_launch(bool pro, int G, int nchunks, int scalar, const RkCoef& rc) {
     constexpr int TPC = L / Plan<L>::E;
     const int nouter = 2 * g_.keep_y + 1;
     const long long blocks = (long long)nchunks * cdiv(nouter, G);
     const size_t lds = (size_t)G * L * CC * sizeof(C);
     if (pro)
-      hipLaunchKernelGGL((k_xinv_prologue<T, L, CC, VEL>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_,
+      hipLaunchKernelGGL((k_xinv_prologue<T, L, CC, GRP>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_,
                          g_, ptrs(q_), ptrs(ws_), tkx_, tky_, tkz_, tw_.at(L), scalar, nchunks, G);
     else
-      hipLaunchKernelGGL((k_xfwd_rk<T, L, CC, VEL>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_, g_,
+      hipLaunchKernelGGL((k_xfwd_rk<T, L, CC, GRP == 0>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_, g_,
                          ptrs(q_), ptrs(dq_), ptrs(nl_), tkx_, tky_, tkz_, tw_.at(L), rc, scalar, nchunks, G);
+  }
+  template <int L, int CC>
+  void xgrp_k(bool pro, int G, int nchunks, int grp, const RkCoef& rc) {
+    if (grp < 0) xgrp_launch<L, CC, 0>(pro, G, nchunks, 0, rc);
+    else if (pro && colloid_ && grp == S_ - 1) xgrp_launch<L, CC, 2>(pro, G, nchunks, grp, rc);
+    else xgrp_launch<L, CC, 1>(pro, G, nchunks, grp, rc);
   }
   template <int L>
   void xgrp_c(bool pro, int G, int nchunks, int grp, const RkCoef& rc) {
-    const bool vel = grp < 0;
     if constexpr (sizeof(T) == 8) {
-      if (cx_ == 8) vel ? xgrp_launch<L, 8, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 8, false>(pro, G, nchunks, grp, rc);
-      else vel ? xgrp_launch<L, 4, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 4, false>(pro, G, nchunks, grp, rc);
+      if (cx_ == 8) xgrp_k<L, 8>(pro, G, nchunks, grp, rc);
+      else xgrp_k<L, 4>(pro, G, nchunks, grp, rc);
     } else {
-      if (cx_ == 16) vel ? xgrp_launch<L, 16, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 16, false>(pro, G, nchunks, grp, rc);
-      else vel ? xgrp_launch<L, 8, true>(pro, G, nchunks, 0, rc) : xgrp_launch<L, 8, false>(pro, G, nchunks, grp, rc);
+      if (cx_ == 16) xgrp_k<L, 16>(pro, G, nchunks, grp, rc);
+      else xgrp_k<L, 8>(pro, G, nchunks, grp, rc);
     }
   }
-  // grp < 0: velocity group; grp = s: scalar s.
+  // grp < 0: velocity group; grp = s: scalar s (the colloid group when s is the colloid).
   void x_fused(bool pro, int grp, const RkCoef& rc) {
     const int L = nx_, CC = cx_;
     const int TPC = L / elems_per_thread(L);
@@ -999,7 +1010,8 @@ class Engine : public EngineBase {
     const int nchunks = kz_lim_ / CC;
     const int nouter = 2 * g_.keep_y + 1;
     const double kept_rows = 2 * g_.keep_x + 1;
-    const int nin = grp < 0 ? 3 : 1, nout = grp < 0 ? 6 : 3;
+    const bool cgrp = pro && colloid_ && grp == S_ - 1;
+    const int nin = grp < 0 ? 3 : (cgrp ? 2 : 1), nout = grp < 0 ? 6 : (cgrp ? 5 : 3);
     const double per_col = pro ? (nin * kept_rows + nout * (double)L)
                                : (nin * (double)L + nin * kept_rows * (2 + (rc.first ? 0 : 1) + (rc.last ? 0 : 1)));
     const double bytes = (double)nouter * nchunks * CC * per_col * sizeof(C);
@@ -1032,6 +1044,8 @@ class Engine : public EngineBase {
     a.kzs = kzs_;
     a.lim = kz_lim_;
     a.S = S_;
+    a.colloid = colloid_;
+    a.alpha = (T)p_.alpha;
     const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
     const double bytes = (double)nx_ * ny_ * (double)kz_lim_ * sizeof(C) * (W_ + F_);
     timed("fft_z_fused", bytes, [&] { GNSS_Z_SWITCH(zfused_launch, a, scale); });
@@ -1064,7 +1078,7 @@ class Engine : public EngineBase {
     } else {
       timed("k_prologue", (double)nb * (F_ + W_) * sizeof(C), [&] {
         hipLaunchKernelGGL(k_prologue<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_),
-                           ptrs(ws_), tkx_, tky_, tkz_, S_, nb);
+                           ptrs(ws_), tkx_, tky_, tkz_, S_, colloid_, nb);
       });
     }
     if (unfused) {
@@ -1072,7 +1086,7 @@ class Engine : public EngineBase {
       for (int w = 0; w < W_; ++w) inv3d(ws_[w], wp_[w], true);
       timed("k_products", (double)nph_ * (W_ + F_) * sizeof(T), [&] {
         hipLaunchKernelGGL(k_products<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, rptrs(wp_),
-                           rptrs(nlp_), S_, nph_);
+                           rptrs(nlp_), S_, colloid_, (T)p_.alpha, nph_);
       });
       for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
     } else {
@@ -1100,6 +1114,7 @@ class Engine : public EngineBase {
 
   // -------------------------------------------------------------------------------------------
   gnss_params p_;
+  int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
   int cp_ = 0, n_cu_ = 256;
   int kz_lim_ = 0;

@@ -83,10 +83,15 @@ void build_initial_condition(const gnss_params& p, std::vector<HostIcEntry>& out
   // jet: U depends on y only -> modes (0, my, 0): Nx Nz sum_j U(y_j) exp(-2 pi i my j / Ny)
   std::vector<double> tw;
   twiddle_table(ny, tw);
-  std::vector<double> prof(ny);
+  // the colloid (last scalar with GNSS_FLAG_COLLOID) has its own thickness theta_c
+  // (initializeColloidKernel_mgpu, initialize.cu:178-194)
+  const bool colloid = (p.flags & GNSS_FLAG_COLLOID) && S >= 2;
+  const double theta_c = p.theta_c > 0 ? p.theta_c : p.theta;
+  std::vector<double> prof(ny), prof_c(ny);
   for (int j = 0; j < ny; ++j) {
     const double y = (double)j * p.ly / (double)ny;
     prof[j] = 0.5 * (1.0 + tanh((0.5 * h - fabs(y - 0.5 * p.ly)) / (2.0 * p.theta)));
+    prof_c[j] = 0.5 * (1.0 + tanh((0.5 * h - fabs(y - 0.5 * p.ly)) / (2.0 * theta_c)));
   }
   // accumulation box of the noise: mx, my in [-K, K], mz in [0, K]; the jet enters at (0, my, 0)
   struct Acc {
@@ -99,17 +104,21 @@ void build_initial_condition(const gnss_params& p, std::vector<HostIcEntry>& out
   auto vidx = [&](int comp, int mx, int my, int mz) {
     return (((size_t)comp * bx + (size_t)(mx + K)) * bx + (size_t)(my + K)) * bz + (size_t)mz;
   };
-  std::vector<double> jr(ny), ji(ny);
+  std::vector<double> jr(ny), ji(ny), cr(ny), ci(ny);
   for (int iy = 0; iy < ny; ++iy) {
-    double re = 0.0, im = 0.0;
+    double re = 0.0, im = 0.0, rc = 0.0, ic = 0.0;
     for (int j = 0; j < ny; ++j) {
       const int m = (int)(((int64_t)iy * j) % ny);
       re += prof[j] * tw[2 * m];
       im += prof[j] * tw[2 * m + 1];
+      rc += prof_c[j] * tw[2 * m];
+      ic += prof_c[j] * tw[2 * m + 1];
     }
     const double sx = (double)nx * (double)nz;
     jr[iy] = re * sx;
     ji[iy] = im * sx;
+    cr[iy] = rc * sx;
+    ci[iy] = ic * sx;
     const int my = iy < ny / 2 ? iy : iy - ny;
     if (my >= -K && my <= K) {
       Acc& a = vel[vidx(0, 0, my, 0)];
@@ -186,7 +195,8 @@ void build_initial_condition(const gnss_params& p, std::vector<HostIcEntry>& out
     for (int iy = 0; iy < ny; ++iy) {
       const int my = iy < ny / 2 ? iy : iy - ny;
       if (!kept(p, 0, my, 0)) continue;
-      out.push_back(HostIcEntry{0, iy, 0, 3 + s, jr[iy], ji[iy]});
+      const bool c = colloid && s == S - 1;
+      out.push_back(HostIcEntry{0, iy, 0, 3 + s, c ? cr[iy] : jr[iy], c ? ci[iy] : ji[iy]});
     }
 }
 

@@ -317,11 +317,18 @@ template <typename T, int M> struct FusedPlan<T, M, 2> {
   static constexpr bool prefetch = false;
 };
 
+// Work rows: u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar, and
+// with a colloid (the last scalar) two more: c itself and lap Z (Z = scalar 0).
+constexpr int MAXF = 3 + 4;           // velocity + up to 4 scalars
+constexpr int MAXW = 6 + 3 * 4 + 2;   // work rows of the largest field set
+
 template <typename T> struct ZFusedArgs {
-  const cx<T>* ws[18];   // u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar
-  cx<T>* nl[7];          // (u x omega)_x,y,z, u . grad phi per scalar
+  const cx<T>* ws[MAXW];
+  cx<T>* nl[MAXF];       // (u x omega)_x,y,z, then the advection term of each scalar
   long long nrows;
   int kzs, lim, S, G;
+  int colloid;           // 1: scalar S - 1 is a colloid drifting with alpha grad(scalar 0)
+  T alpha;
 };
 
 template <typename V> __device__ __forceinline__ void fma_pair(V& acc, const V& a, const V& b) {
@@ -339,9 +346,10 @@ template <typename V> __device__ __forceinline__ V mul_pair(const V& a, const V&
   return r;
 }
 
-// Fused z pass of one substage: C2R of the 6 + 3S work rows, the physical products u x omega and
-// u . grad phi (crossProductKernel_mgpu / dotKernel_mgpu, solver.cu:139-164, 342-356), and R2C
-// of the 3 + S products — the physical fields never leave registers.  Rows hold kz < lim only.
+// Fused z pass of one substage: C2R of the work rows, the physical products u x omega and
+// u . grad phi (crossProductKernel_mgpu / dotKernel_mgpu, solver.cu:139-164, 342-356; the colloid
+// term of colloidAdvectionKernel_mgpu, solver.cu:374-391), and R2C of the 3 + S products — the
+// physical fields never leave registers.  Rows hold kz < lim only.
 // Fields move through the radix stages three at a time (u, v, w; omega; products; grad phi),
 // and in FP32 the HBM loads of the next three rows are issued before the current ones transform.
 // Each row owns FUSED_B LDS regions of RS entries.
@@ -373,9 +381,13 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
 #pragma unroll
     for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
   }
-  if (PREFETCH && a.S > 0) {
+  // Plain scalars; with a colloid, scalar 0 (its carrier Z) goes last so that the velocity
+  // registers can then take the drift velocity u + alpha grad Z.
+  const int P = a.S - a.colloid;
+  const int s_first = (a.colloid && P > 1) ? 1 : 0;
+  if (PREFETCH && P > 0) {
 #pragma unroll
-    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + b] + off, active, a.lim, t, r0[b]);
+    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s_first + b] + off, active, a.lim, t, r0[b]);
   }
   Z::template c2r<3>(r1, lds, idx, t, twM, twN, scale, om);  // omega
   // u x omega, in place of omega
@@ -391,8 +403,9 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
     cx<T>* dst[3] = {a.nl[0] + off, a.nl[1] + off, a.nl[2] + off};
     Z::template r2c<3>(om, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
   }
-  for (int s = 0; s < a.S; ++s) {
-    if (!PREFETCH || s > 0) {
+  for (int j = 0; j < P; ++j) {
+    const int s = a.colloid ? (j + 1 < P ? j + 1 : 0) : j;
+    if (!PREFETCH || j > 0) {
 #pragma unroll
       for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s + b] + off, active, a.lim, t, r0[b]);
     }
@@ -404,7 +417,35 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
       n[0][e] = mk<T>(u.x * om[0][e].x + v.x * om[1][e].x + w.x * om[2][e].x,
                       u.y * om[0][e].y + v.y * om[1][e].y + w.y * om[2][e].y);
     }
+    if (a.colloid && s == 0) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int e = 0; e < E; ++e) fma_pair(vel[b][e], mk<T>(a.alpha, a.alpha), om[b][e]);
+    }
     cx<T>* dst[1] = {a.nl[3 + s] + off};
+    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+  }
+  if (a.colloid) {
+    // colloidAdvectionKernel_mgpu (solver.cu:374-391): u.grad c + alpha (lap Z c + grad Z . grad c)
+    // = (u + alpha grad Z) . grad c + alpha lap Z c, with u + alpha grad Z already in vel.
+    const int sc = a.S - 1;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * sc + b] + off, active, a.lim, t, r0[b]);
+    Z::template c2r<3>(r0, lds, idx, t, twM, twN, scale, om);  // grad c
+    cx<T> n[1][E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const cx<T> u = vel[0][e], v = vel[1][e], w = vel[2][e];
+      n[0][e] = mk<T>(u.x * om[0][e].x + v.x * om[1][e].x + w.x * om[2][e].x,
+                      u.y * om[0][e].y + v.y * om[1][e].y + w.y * om[2][e].y);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) Z::load(a.ws[6 + 3 * a.S + b] + off, active, a.lim, t, r0[b]);
+    Z::template c2r<2>(r0, lds, idx, t, twM, twN, scale, om);  // c, lap Z
+#pragma unroll
+    for (int e = 0; e < E; ++e) fma_pair(n[0][e], mk<T>(a.alpha * om[1][e].x, a.alpha * om[1][e].y), om[0][e]);
+    cx<T>* dst[1] = {a.nl[3 + sc] + off};
     Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
   }
 }

@@ -5,9 +5,6 @@
 
 namespace gnss {
 
-constexpr int MAXF = 3 + 4;           // velocity + up to 4 scalars
-constexpr int MAXW = 6 + 3 * 4;       // u, v, w, omega, grad phi per scalar
-
 template <typename T> struct CPtrs { cx<T>* p[MAXW]; };
 template <typename T> struct RPtrs { T* p[MAXW]; };
 
@@ -49,10 +46,12 @@ template <typename V, typename T> __device__ __forceinline__ V ik(T k, V f) {
 }
 
 // Spectral prologue of a substage: u, v, w, omega = i k x u^ (calcOmega*Kernel_mgpu,
-// solver.cu:68-114) and grad phi = i k phi^ (gradientKernel_mgpu, solver.cu:233-256).
+// solver.cu:68-114) and grad phi = i k phi^ (gradientKernel_mgpu, solver.cu:233-256); with a
+// colloid (the last scalar) also c^ itself and lap Z^ = -k^2 Z^ of its carrier Z = scalar 0
+// (divergenceKernel_mgpu of the gradient, solver.cu:276-293, scalarAdvection solver.cu:423).
 template <typename T>
 __global__ void k_prologue(Geom g, Box b, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv,
-                           const T* __restrict__ kyv, const T* __restrict__ kzv, int S,
+                           const T* __restrict__ kyv, const T* __restrict__ kzv, int S, int colloid,
                            long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -73,12 +72,20 @@ __global__ void k_prologue(Geom g, Box b, CPtrs<T> q, CPtrs<T> ws, const T* __re
     ws.p[7 + 3 * s][o] = ik(ky, ph);
     ws.p[8 + 3 * s][o] = ik(kz, ph);
   }
+  if (colloid) {
+    const T k2 = kx * kx + ky * ky + kz * kz;
+    const cx<T> zh = q.p[3][o];
+    ws.p[6 + 3 * S][o] = q.p[2 + S][o];
+    ws.p[7 + 3 * S][o] = mk<T>(-k2 * zh.x, -k2 * zh.y);
+  }
 }
 
 // Physical products: u x w (crossProductKernel_mgpu, solver.cu:139-164, with the sign of
-// computeRHSKernel_mgpu folded in) and u . grad phi (dotKernel_mgpu, solver.cu:342-356).
+// computeRHSKernel_mgpu folded in), u . grad phi (dotKernel_mgpu, solver.cu:342-356) and, for a
+// colloid c (the last scalar), u . grad c + alpha (lap Z c + grad Z . grad c)
+// (colloidAdvectionKernel_mgpu, solver.cu:374-391).
 template <typename T>
-__global__ void k_products(RPtrs<T> p, RPtrs<T> nl, int S, long long n) {
+__global__ void k_products(RPtrs<T> p, RPtrs<T> nl, int S, int colloid, T alpha, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T u = p.p[0][i], v = p.p[1][i], w = p.p[2][i];
@@ -86,7 +93,14 @@ __global__ void k_products(RPtrs<T> p, RPtrs<T> nl, int S, long long n) {
   nl.p[0][i] = v * oz - w * oy;
   nl.p[1][i] = w * ox - u * oz;
   nl.p[2][i] = u * oy - v * ox;
-  for (int s = 0; s < S; ++s) nl.p[3 + s][i] = u * p.p[6 + 3 * s][i] + v * p.p[7 + 3 * s][i] + w * p.p[8 + 3 * s][i];
+  for (int s = 0; s < S - colloid; ++s)
+    nl.p[3 + s][i] = u * p.p[6 + 3 * s][i] + v * p.p[7 + 3 * s][i] + w * p.p[8 + 3 * s][i];
+  if (colloid) {
+    const int c0 = 6 + 3 * (S - 1);
+    const T cx = p.p[c0][i], cy = p.p[c0 + 1][i], cz = p.p[c0 + 2][i];
+    const T c = p.p[6 + 3 * S][i], lz = p.p[7 + 3 * S][i];
+    nl.p[2 + S][i] = alpha * (lz * c + cx * p.p[6][i] + cy * p.p[7][i] + cz * p.p[8][i]) + (u * cx + v * cy + w * cz);
+  }
 }
 
 template <typename T> __device__ __forceinline__ T dexp(T x);
@@ -425,15 +439,17 @@ __global__ void k_finite(Geom g, CPtrs<T> q, int F, int* __restrict__ flag) {
 // Inverse x pass of the work fields of one field group, fed straight from the state.  Per
 // (ky kept, kz chunk) tile it loads the group's state columns once (kept kx rows only), forms
 // each work field in registers and transforms it:
-//   VEL:     u, v, w -> u, v, w, omega = i k x u   (calcOmega*Kernel_mgpu, solver.cu:68-114)
-//   scalar:  phi_s   -> i k phi_s                  (gradientKernel_mgpu, solver.cu:233-256)
+//   GRP 0 (velocity): u, v, w -> u, v, w, omega = i k x u  (calcOmega*Kernel_mgpu, solver.cu:68-114)
+//   GRP 1 (scalar):   phi_s   -> i k phi_s                 (gradientKernel_mgpu, solver.cu:233-256)
+//   GRP 2 (colloid):  c, Z    -> i k c, c, -k^2 Z          (scalarAdvection, solver.cu:419-437)
 // Replaces k_prologue and the separate inverse x passes of those work fields.
-template <typename T, int L, int C, bool VEL>
+template <typename T, int L, int C, int GRP>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
 k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, const T* __restrict__ kyv,
                 const T* __restrict__ kzv, const cx<T>* __restrict__ tw, int scalar, int nchunks, int G) {
-  constexpr int NIN = VEL ? 3 : 1;
-  constexpr int NOUT = VEL ? 6 : 3;
+  constexpr bool VEL = GRP == 0;
+  constexpr int NIN = VEL ? 3 : (GRP == 2 ? 2 : 1);
+  constexpr int NOUT = VEL ? 6 : (GRP == 2 ? 5 : 3);
   constexpr int E = Plan<L>::E;
   constexpr int TPC = L / E;
   extern __shared__ unsigned char smem_raw[];
@@ -459,7 +475,10 @@ k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, cons
     const int p = t + e * TPC;
     const bool ld = active && row_kept(p, L, g.keep_x);
 #pragma unroll
-    for (int f = 0; f < NIN; ++f) qv[f][e] = ld ? q.p[in0 + f][col + p * estride] : mk<T>(0, 0);
+    for (int f = 0; f < NIN; ++f) {
+      const int fi = f == 0 ? in0 : (GRP == 2 ? 3 : in0 + f);  // the colloid group also reads Z
+      qv[f][e] = ld ? q.p[fi][col + p * estride] : mk<T>(0, 0);
+    }
   }
   const LdsCol idx{gg * L * C + c, C};
 #pragma unroll 1
@@ -474,6 +493,11 @@ k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, cons
         else if (w == 3) v[e] = csub(ik(ky, ww), ik(kz, vv));
         else if (w == 4) v[e] = csub(ik(kz, u), ik(kx, ww));
         else v[e] = csub(ik(kx, vv), ik(ky, u));
+      } else if (GRP == 2 && w == 3) {
+        v[e] = qv[0][e];
+      } else if (GRP == 2 && w == 4) {
+        const T k2 = kx * kx + ky * ky + kz * kz;
+        v[e] = mk<T>(-k2 * qv[NIN - 1][e].x, -k2 * qv[NIN - 1][e].y);
       } else {
         v[e] = ik(w == 0 ? kx : (w == 1 ? ky : kz), qv[0][e]);
       }

@@ -31,7 +31,7 @@ ENGINE_LIB = os.path.join(HERE, "lib", "libgnss_hip.so")
 MAX_SCALARS = 4
 FP64, FP32 = 0, 1
 SPECTRAL, PHYSICAL = 0, 1
-FLAG_MFMA_R16, FLAG_UNFUSED = 1, 2
+FLAG_MFMA_R16, FLAG_UNFUSED, FLAG_COLLOID = 1, 2, 4
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_ALLOC", 4: "ERR_STATE",
           5: "ERR_NUMERIC", 6: "ERR_UNSUPPORTED"}
@@ -58,6 +58,8 @@ class Params(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("precision", ctypes.c_int),
         ("flags", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("theta_c", ctypes.c_double),
     ]
 
 
@@ -160,6 +162,8 @@ class Config:
     dt: float = 1e-3
     precision: int = FP64
     flags: int = 0
+    alpha: float = 0.0       # colloid drift coefficient (with FLAG_COLLOID: last scalar is the colloid)
+    theta_c: float = 0.0     # colloid profile thickness (0: theta)
 
     def to_params(self) -> Params:
         p = Params()
@@ -177,6 +181,8 @@ class Config:
         p.dt = float(self.dt)
         p.precision = int(self.precision)
         p.flags = int(self.flags)
+        p.alpha = float(self.alpha)
+        p.theta_c = float(self.theta_c)
         return p
 
 
@@ -398,6 +404,17 @@ def config(name: str, **over) -> Config:
     d = dict(base)
     d.update(table[name])
     d.update(over)
+    return Config(**d)
+
+
+def with_colloid(cfg: Config, alpha: float = 0.0033, sc_c: float = 0.7, theta_c: float = 0.0) -> Config:
+    """`cfg` plus a colloid as the last scalar, carried by u + alpha grad(scalar 0); defaults are
+    the reference's alpha, Sc_c and theta_c = theta (dnsparams.h:28-33)."""
+    d = dict(cfg.__dict__)
+    d["sc"] = tuple(cfg.sc) + (sc_c,)
+    d["flags"] = int(cfg.flags) | FLAG_COLLOID
+    d["alpha"] = float(alpha)
+    d["theta_c"] = float(theta_c)
     return Config(**d)
 
 

@@ -6,8 +6,9 @@
   jet1024   FP32 at 1024^3 (50 steps) with the VALU radix stages and with the split-FP16 MFMA
             radix-16 stage; errors against each other and against FP64 at 512^3 (same physical
             parameters) compared on the 512^3 modes
+  colloid512  the 512^3 jet plus the colloid scalar, FP32 and FP64, 100 steps
 
-usage: python gnss-1_amd/run_configs.py {jet256,ensemble,jet1024} [--steps K]
+usage: python gnss-1_amd/run_configs.py {jet256,ensemble,jet1024,colloid512} [--steps K]
 """
 import argparse
 import json
@@ -118,13 +119,37 @@ def run_jet1024(steps):
     return out
 
 
+def run_colloid512(steps):
+    """The 512^3 jet with the reference's colloid added as a second scalar (alpha 0.0033,
+    Sc_c 0.7): FP32 and FP64 rates and the FP32 error against FP64 after `steps` steps."""
+    out = {"workload": "jet512 + colloid (alpha 0.0033, Sc_c 0.7)", "steps": steps}
+    fields = {}
+    for name, prec in (("fp32", gnss.FP32), ("fp64", gnss.FP64)):
+        cfg = gnss.with_colloid(gnss.config("jet512", precision=prec))
+        s = gnss.Solver(cfg).setup()
+        s.step(1)
+        t0 = time.perf_counter()
+        s.step(steps - 1)
+        dt = time.perf_counter() - t0
+        out[f"{name}_steps_per_s"] = (steps - 1) / dt
+        st = s.stats(False, False)[0]
+        out[f"{name}_stats"] = {k: st[k] for k in ("ke", "epsilon", "divergence")}
+        out[f"{name}_chi"] = list(st["chi"]) if "chi" in st else None
+        fields[name] = [s.get_spectral(f) for f in range(5)]
+        log(f"[colloid512] {name}: {out[f'{name}_steps_per_s']:.2f} steps/s")
+        s.close()
+    out["fp32_vs_fp64_rel_l2"] = [rel(fields["fp32"][f], fields["fp64"][f]) for f in range(5)]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["jet256", "ensemble", "jet1024"])
+    ap.add_argument("which", choices=["jet256", "ensemble", "jet1024", "colloid512"])
     ap.add_argument("--steps", type=int, default=0)
     a = ap.parse_args()
-    steps = a.steps or {"jet256": 200, "ensemble": 200, "jet1024": 50}[a.which]
-    fn = {"jet256": run_jet256, "ensemble": run_ensemble, "jet1024": run_jet1024}[a.which]
+    steps = a.steps or {"jet256": 200, "ensemble": 200, "jet1024": 50, "colloid512": 100}[a.which]
+    fn = {"jet256": run_jet256, "ensemble": run_ensemble, "jet1024": run_jet1024,
+          "colloid512": run_colloid512}[a.which]
     print(json.dumps(fn(steps)))
 
 

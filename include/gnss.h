@@ -25,7 +25,8 @@
  *                    (fftfuncs.cu:288-359, cudafuncs.cu:19-31).
  *   mask      uint8  [ky][kx][kz], 1 = kept by the dealiasing rule.
  *
- * Field ids: 0 = u, 1 = v, 2 = w, 3 + s = passive scalar s (s < n_scalars).
+ * Field ids: 0 = u, 1 = v, 2 = w, 3 + s = passive scalar s (s < n_scalars); with
+ * GNSS_FLAG_COLLOID the last scalar is the colloid.
  */
 #ifndef GNSS_H
 #define GNSS_H
@@ -52,7 +53,11 @@ enum gnss_precision { GNSS_FP64 = 0, GNSS_FP32 = 1 };
 enum gnss_space { GNSS_SPECTRAL = 0, GNSS_PHYSICAL = 1 };
 enum gnss_flags {
   GNSS_FLAG_MFMA_R16 = 1,   /* FP32 form: radix-16 DFT stages on MFMA with split-FP16 operands */
-  GNSS_FLAG_UNFUSED = 2     /* engine: run the step as separate transform/pointwise kernels */
+  GNSS_FLAG_UNFUSED = 2,    /* engine: run the step as separate transform/pointwise kernels */
+  GNSS_FLAG_COLLOID = 4     /* the last scalar is a colloid c carried by u + alpha grad(scalar 0):
+                               its nonlinear term is u.grad c + alpha (lap(Z) c + grad Z . grad c),
+                               Z = scalar 0 (colloidAdvectionKernel_mgpu, solver.cu:374-405, and
+                               scalarAdvection, solver.cu:407-459); needs n_scalars >= 2 */
 };
 
 /* Runtime replacement of the reference's compile-time configuration (dnsparams.h:9-41). */
@@ -70,6 +75,9 @@ typedef struct gnss_params {
   double dt;                 /* fixed time step (reference dt, dnsparams.h:23) */
   int precision;             /* GNSS_FP64 or GNSS_FP32 */
   int flags;                 /* gnss_flags */
+  double alpha;              /* colloid drift coefficient (reference alpha, dnsparams.h:33) */
+  double theta_c;            /* colloid initial-profile thickness (reference theta_c,
+                                dnsparams.h:28; 0 means theta) */
 } gnss_params;
 
 /* Integral statistics (reference statistics struct, struct_def.h:13-29; calcTurbStats_mgpu,

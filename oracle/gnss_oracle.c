@@ -15,6 +15,10 @@
  *   cross product            u x w in physical space  (solver.cu:139-164, sign fixed by
  *                            computeRHSKernel_mgpu, solver.cu:504-511)
  *   scalar advection         u . grad(phi), convective form  (solver.cu:233-256, 342-356, 407-432)
+ *   colloid (optional)       last scalar c with drift alpha grad Z (Z = scalar 0):
+ *                            u . grad c + alpha (lap Z c + grad Z . grad c)
+ *                            (solver.cu:276-293, 374-459), own profile thickness theta_c
+ *                            (initialize.cu:178-194)
  *   projection               rhs_i = (delta_ij - k_i k_j/k^2) (u x w)^_j, k = 0 zeroed;
  *                            scalar rhs = -(u.grad phi)^  (solver.cu:461-521)
  *   time integration         DEPARTURE: low-storage RK3 (Williamson 1980, A = 0,-5/9,-153/128,
@@ -34,7 +38,9 @@
  * (FFT); closed forms for wavenumbers/mask (enumeration), single-mode exact viscous decay (IF),
  * 2D Taylor-Green exact decay (whole step), projection invariants, third-order convergence
  * (RK3), inviscid energy conservation, single-mode statistics, Hermitian/divergence/amplitude
- * properties of the initial condition.
+ * properties of the initial condition; the colloid term by the exact steady state
+ * c = exp(alpha Z / D_c) of a frozen carrier, its uniform-flow translation, and the plain-scalar
+ * limit alpha = 0.
  *
  * FFT backends: a self-contained Stockham mixed-radix FFT (default), or the FFTW3 API resolved
  * at run time from libfftw3 / libmkl_rt (used for the threaded CPU baseline).
@@ -242,7 +248,8 @@ EXPORT int gnss_oracle_max_threads(void) {
 
 struct gnss_ctx {
   gnss_params p;
-  int nx, ny, nz, nzh, S, F;
+  int nx, ny, nz, nzh, S, F, W;
+  int colloid;      /* 1: scalar S - 1 is a colloid carried by u + alpha grad(scalar 0) */
   size_t nsp, nph;
   double nu, diff[GNSS_MAX_SCALARS];
   fftplan px, py, pz;
@@ -252,8 +259,8 @@ struct gnss_ctx {
   cplx** q;         /* F spectral state, internal [kx][ky][kz] */
   cplx** dq;        /* F RK register */
   double** phys;    /* F physical buffers [x][y][z] */
-  cplx** wsp;       /* work spectral (6 + 3S) */
-  double** wph;     /* work physical (6 + 3S) */
+  cplx** wsp;       /* work spectral (W = 6 + 3S, + 2 with a colloid) */
+  double** wph;     /* work physical (W) */
   cplx** nl;        /* 3 + S nonlinear terms (spectral) */
   double** nlph;    /* 3 + S nonlinear terms (physical) */
   cplx* tmp;        /* one spectral scratch */
@@ -402,7 +409,7 @@ static void* xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
 static void ctx_free(gnss_ctx* c) {
   if (!c) return;
-  int W = 6 + 3 * c->S;
+  const int W = c->W;
   for (int i = 0; i < c->F; ++i) {
     if (c->q) free(c->q[i]);
     if (c->dq) free(c->dq[i]);
@@ -440,11 +447,17 @@ EXPORT int gnss_create(const gnss_params* p, gnss_ctx** out) {
     if (!(p->sc[s] > 0)) return set_err(NULL, GNSS_ERR_ARG, "sc[%d] must be > 0", s);
   if (p->precision != GNSS_FP64)
     return set_err(NULL, GNSS_ERR_UNSUPPORTED, "the oracle computes in FP64 only");
+  if ((p->flags & GNSS_FLAG_COLLOID) && p->n_scalars < 2)
+    return set_err(NULL, GNSS_ERR_ARG, "GNSS_FLAG_COLLOID needs n_scalars >= 2 (scalar 0 carries the colloid)");
+  if ((p->flags & GNSS_FLAG_COLLOID) && !isfinite(p->alpha))
+    return set_err(NULL, GNSS_ERR_ARG, "alpha must be finite");
   gnss_ctx* c = (gnss_ctx*)xcalloc(1, sizeof(gnss_ctx));
   if (!c) return set_err(NULL, GNSS_ERR_ALLOC, "out of host memory");
   c->p = *p;
   c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nzh = p->nz / 2 + 1;
   c->S = p->n_scalars; c->F = 3 + c->S;
+  c->colloid = (p->flags & GNSS_FLAG_COLLOID) ? 1 : 0;
+  c->W = 6 + 3 * c->S + 2 * c->colloid;
   c->nsp = (size_t)c->nx * c->ny * c->nzh;
   c->nph = (size_t)c->nx * c->ny * c->nz;
   c->nu = 1.0 / p->re;
@@ -453,7 +466,7 @@ EXPORT int gnss_create(const gnss_params* p, gnss_ctx** out) {
     ctx_free(c);
     return set_err(NULL, GNSS_ERR_ALLOC, "out of host memory (plans)");
   }
-  const int F = c->F, W = 6 + 3 * c->S;
+  const int F = c->F, W = c->W;
   c->q = (cplx**)xcalloc(F, sizeof(cplx*));
   c->dq = (cplx**)xcalloc(F, sizeof(cplx*));
   c->phys = (double**)xcalloc(F, sizeof(double*));
@@ -601,9 +614,12 @@ static cplx noise_gauss(uint64_t seed, int comp, int mx, int my, int mz) {
   return r * cos(a) + I * (r * sin(a));
 }
 
-static double jet_profile(const gnss_params* p, double y) {
+/* jet and scalar profile (initializeVelocityKernel_mgpu / initializeScalarKernel_mgpu,
+ * initialize.cu:114-176); the colloid's uses its own thickness theta_c
+ * (initializeColloidKernel_mgpu, initialize.cu:178-194) */
+static double jet_profile(const gnss_params* p, double theta, double y) {
   const double h = p->jet_h > 0 ? p->jet_h : 1.0;
-  return 0.5 * (1.0 + tanh((0.5 * h - fabs(y - 0.5 * p->ly)) / (2.0 * p->theta)));
+  return 0.5 * (1.0 + tanh((0.5 * h - fabs(y - 0.5 * p->ly)) / (2.0 * theta)));
 }
 
 EXPORT int gnss_init_jet(gnss_ctx* c) {
@@ -618,21 +634,29 @@ EXPORT int gnss_init_jet(gnss_ctx* c) {
   }
   /* jet: field depends on y only -> nonzero only at kx = kz = 0; direct DFT along y */
   cplx* tw = (cplx*)malloc(sizeof(cplx) * ny);
-  double* prof = (double*)malloc(sizeof(double) * ny);
+  double* prof = (double*)malloc(sizeof(double) * 2 * ny);
   if (!tw || !prof) { free(tw); free(prof); return set_err(c, GNSS_ERR_ALLOC, "out of host memory"); }
+  double* prof_c = prof + ny;
+  const double theta_c = c->p.theta_c > 0 ? c->p.theta_c : c->p.theta;
   twiddle_table(tw, ny);
-  for (int j = 0; j < ny; ++j) prof[j] = jet_profile(&c->p, (double)j * c->p.ly / (double)ny);
+  for (int j = 0; j < ny; ++j) {
+    prof[j] = jet_profile(&c->p, c->p.theta, (double)j * c->p.ly / (double)ny);
+    prof_c[j] = jet_profile(&c->p, theta_c, (double)j * c->p.ly / (double)ny);
+  }
   for (int iy = 0; iy < ny; ++iy) {
-    double re = 0.0, im = 0.0;
+    double re = 0.0, im = 0.0, rc = 0.0, ic = 0.0;
     for (int j = 0; j < ny; ++j) {
       int m = (int)(((int64_t)iy * j) % ny);
       re += prof[j] * creal(tw[m]);
       im += prof[j] * cimag(tw[m]);
+      rc += prof_c[j] * creal(tw[m]);
+      ic += prof_c[j] * cimag(tw[m]);
     }
     const double sx = (double)nx * (double)c->nz;
     cplx v = (re * sx) + I * (im * sx);
     c->q[0][(size_t)iy * nzh] = v;
     for (int s = 0; s < c->S; ++s) c->q[3 + s][(size_t)iy * nzh] = v;
+    if (c->colloid) c->q[2 + c->S][(size_t)iy * nzh] = (rc * sx) + I * (ic * sx);
   }
   free(tw);
   free(prof);
@@ -693,7 +717,8 @@ static const double RK_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 static const double RK_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 static const double RK_C[4] = {0.0, 1.0 / 3.0, 3.0 / 4.0, 1.0};
 
-/* Nonlinear terms of the state q (spectral, masked): nl[0..2] = (u x w)^, nl[3+s] = (u.grad phi_s)^ */
+/* Nonlinear terms of the state q (spectral, masked): nl[0..2] = (u x w)^, nl[3+s] = (u.grad phi_s)^
+ * (for the colloid: the drift-advection term above) */
 static void nonlinear(gnss_ctx* c) {
   const int nx = c->nx, ny = c->ny, nzh = c->nzh, S = c->S;
   cplx** w = c->wsp;
@@ -718,9 +743,16 @@ static void nonlinear(gnss_ctx* c) {
         w[7 + 3 * s][i] = I * (ky * ph);
         w[8 + 3 * s][i] = I * (kz * ph);
       }
+      if (c->colloid) {
+        /* c itself and lap Z = div(grad Z) = -k^2 Z (divergenceKernel_mgpu, solver.cu:276-293,
+         * applied to grad Z in scalarAdvection, solver.cu:422-423) */
+        w[6 + 3 * S][i] = c->q[2 + S][i];
+        w[7 + 3 * S][i] = -(kx * kx + ky * ky + kz * kz) * c->q[3][i];
+      }
     }
   }
-  const int W = 6 + 3 * S;
+  const int W = c->W, P = S - c->colloid;
+  const double alpha = c->p.alpha;
   for (int f = 0; f < W; ++f) c2r3d(c, w[f], c->wph[f]);
   double** p = c->wph;
 #pragma omp parallel for schedule(static)
@@ -730,8 +762,16 @@ static void nonlinear(gnss_ctx* c) {
     c->nlph[0][i] = v * oz - ww * oy;
     c->nlph[1][i] = ww * ox - u * oz;
     c->nlph[2][i] = u * oy - v * ox;
-    for (int s = 0; s < S; ++s)
+    for (int s = 0; s < P; ++s)
       c->nlph[3 + s][i] = u * p[6 + 3 * s][i] + v * p[7 + 3 * s][i] + ww * p[8 + 3 * s][i];
+    if (c->colloid) {
+      /* colloidAdvectionKernel_mgpu (solver.cu:374-391): with udp = grad Z and divVdp = lap Z,
+       * alpha (divVdp c + grad c . udp) + u . grad c */
+      const int g = 6 + 3 * P;
+      const double cx = p[g][i], cy = p[g + 1][i], cz = p[g + 2][i];
+      const double cc = p[6 + 3 * S][i], lz = p[7 + 3 * S][i];
+      c->nlph[2 + S][i] = alpha * (lz * cc + cx * p[6][i] + cy * p[7][i] + cz * p[8][i]) + (u * cx + v * cy + ww * cz);
+    }
   }
   for (int f = 0; f < 3 + S; ++f) r2c3d(c, c->nlph[f], c->nl[f]);
 }
