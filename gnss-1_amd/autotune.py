@@ -1,8 +1,9 @@
 """Autotuner driver: times the engine's compiled pass variants for each workload on the GPU and
 writes the winning choices to tuning/gnss_tuning.json, which gnss_create reads.
 
-Tuned per (precision, grid, scalars): the column tile width of the strided x and y passes
-(the transpose tile of the one-GPU slab exchange) and the radix plan of the fused z pass.
+Tuned per (precision, grid, scalars, colloid): the column tile widths of the strided x and y
+passes, persistent or one-block-per-tile column passes, the radix plan of the fused z pass, and
+whether (and with which tile) the x passes are fused with the spectral prologue and RK3 update.
 
 usage: python gnss-1_amd/autotune.py [--reps 5] [--out gnss-1_amd/tuning/gnss_tuning.json] [--big]
 """
@@ -27,6 +28,8 @@ def cases(big: bool):
     out.append(gnss.config("jet256", precision=gnss.FP32))
     out.append(gnss.config("jet512", precision=gnss.FP32))
     out.append(gnss.config("jet512"))
+    out.append(gnss.with_colloid(gnss.config("jet512", precision=gnss.FP32)))
+    out.append(gnss.with_colloid(gnss.config("jet512")))
     if big:
         out.append(gnss.config("jet1024", precision=gnss.FP32))
     return out

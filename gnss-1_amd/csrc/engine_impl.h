@@ -44,6 +44,7 @@ class Engine : public EngineBase {
     W_ = 6 + 3 * S_ + 2 * colloid_;
     cx_ = col_chunk(nx_);
     cy_ = col_chunk(ny_);
+    xc_ = cx_;
     zf_ = 0;
     // kz padding and the pruned kz limit use the widest column tile any variant may choose
     const int cmax = sizeof(T) == 8 ? 8 : 16;
@@ -389,11 +390,11 @@ class Engine : public EngineBase {
   }
   std::string tuning_string() const {
     char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d", cx_, cy_, zf_, xf_, cp_);
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d", cx_, cy_, zf_, xf_, cp_, xc_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_;
     const char* p = spec.c_str();
     const char* q;
     if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
@@ -401,13 +402,16 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "cx=")) || (q = strstr(p, "\"cx\": "))) cx = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cy=")) || (q = strstr(p, "\"cy\": "))) cy = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 2) || xf < 0 || xf > 1 || cp < 0 || cp > 1)
+    if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 2) || xf < 0 || xf > 1 ||
+        cp < 0 || cp > 1 || !xchunk_ok(xc))
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
     zf_ = zf;
     xf_ = xf;
     cp_ = cp;
+    xc_ = xc;
   }
 
   std::string autotune(int reps) override {
@@ -500,16 +504,22 @@ class Engine : public EngineBase {
           GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
           GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), dq_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
         }
-        int best = xf_;
+        int best = xf_, best_xc = xc_;
         double best_ms = 1e30;
         for (int v = 0; v <= 1; ++v) {
-          xf_ = v;
-          const double ms = time_it([&] { substage(1); });
-          snprintf(buf, sizeof buf, ", \"xf=%d\": %.5f", v, ms);
-          rep += buf;
-          if (ms < best_ms) { best_ms = ms; best = v; }
+          for (int c : {4, 8, 16}) {
+            if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once
+            xf_ = v;
+            if (v) xc_ = c;
+            const double ms = time_it([&] { substage(1); });
+            if (v) snprintf(buf, sizeof buf, ", \"xf=1 xc=%d\": %.5f", c, ms);
+            else snprintf(buf, sizeof buf, ", \"xf=0\": %.5f", ms);
+            rep += buf;
+            if (ms < best_ms) { best_ms = ms; best = v; best_xc = xc_; }
+          }
         }
         xf_ = best;
+        xc_ = best_xc;
         for (int f = 0; f < F_; ++f) {
           GNSS_HIP(hipMemcpyAsync(q_[f], snap[2 * f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
           GNSS_HIP(hipMemcpyAsync(dq_[f], snap[2 * f + 1], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
@@ -529,6 +539,10 @@ class Engine : public EngineBase {
   }
 
  private:
+  // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
+  bool xchunk_ok(int c) const {
+    return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(nx_, c);
+  }
   bool chunk_ok(int L, int c) const {
     if (sizeof(T) == 8 ? (c != 4 && c != 8) : (c != 8 && c != 16)) return false;
     return (long long)L * c * (long long)sizeof(C) <= LDS_MAX;
@@ -994,16 +1008,16 @@ class Engine : public EngineBase {
   template <int L>
   void xgrp_c(bool pro, int G, int nchunks, int grp, const RkCoef& rc) {
     if constexpr (sizeof(T) == 8) {
-      if (cx_ == 8) xgrp_k<L, 8>(pro, G, nchunks, grp, rc);
+      if (xc_ == 8) xgrp_k<L, 8>(pro, G, nchunks, grp, rc);
       else xgrp_k<L, 4>(pro, G, nchunks, grp, rc);
     } else {
-      if (cx_ == 16) xgrp_k<L, 16>(pro, G, nchunks, grp, rc);
+      if (xc_ == 16) xgrp_k<L, 16>(pro, G, nchunks, grp, rc);
       else xgrp_k<L, 8>(pro, G, nchunks, grp, rc);
     }
   }
   // grp < 0: velocity group; grp = s: scalar s (the colloid group when s is the colloid).
   void x_fused(bool pro, int grp, const RkCoef& rc) {
-    const int L = nx_, CC = cx_;
+    const int L = nx_, CC = xc_;
     const int TPC = L / elems_per_thread(L);
     int G = std::max(1, 256 / (CC * TPC));
     while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
@@ -1116,7 +1130,7 @@ class Engine : public EngineBase {
   gnss_params p_;
   int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
-  int cp_ = 0, n_cu_ = 256;
+  int cp_ = 0, xc_ = 8, n_cu_ = 256;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};
