@@ -255,9 +255,11 @@ GNSS_API int gnss_set_tuning(gnss_ctx* c, const char* spec) {
 // ---------------------------------------------------------------------------------------------
 // Raw field files in the reference's format (iofuncs.cu:329-358 writer, 462-497 reader).
 // ---------------------------------------------------------------------------------------------
-static std::string field_name(int f) {
+// u v w, z for the scalar, c for the colloid (iofuncs.cu:262-327), further scalars z1, z2, ...
+static std::string field_name(const gnss_ctx* c, int f) {
   static const char* vel[3] = {"u", "v", "w"};
   if (f < 3) return vel[f];
+  if (c->eng->is_colloid(f)) return "c";
   return f == 3 ? std::string("z") : "z" + std::to_string(f - 3);
 }
 
@@ -293,7 +295,7 @@ GNSS_API int gnss_save_fields(gnss_ctx* c, const char* dir, int iter) {
     for (int f = 0; f < sz.n_fields; ++f) {
       c->eng->inverse(f);
       c->eng->get_field(f, GNSS_PHYSICAL, buf.data());
-      write_raw(std::string(dir) + "/" + field_name(f) + "." + std::to_string(iter), buf.data(), sz.n_physical);
+      write_raw(std::string(dir) + "/" + field_name(c, f) + "." + std::to_string(iter), buf.data(), sz.n_physical);
     }
     double t;
     int64_t st;
@@ -313,7 +315,7 @@ GNSS_API int gnss_load_fields(gnss_ctx* c, const char* dir, int iter) {
     c->eng->sizes(&sz);
     std::vector<double> buf((size_t)sz.n_physical);
     for (int f = 0; f < sz.n_fields; ++f) {
-      read_raw(std::string(dir) + "/" + field_name(f) + "." + std::to_string(iter), buf.data(), sz.n_physical);
+      read_raw(std::string(dir) + "/" + field_name(c, f) + "." + std::to_string(iter), buf.data(), sz.n_physical);
       c->eng->set_field(f, GNSS_PHYSICAL, buf.data());
       c->eng->forward(f);
     }
@@ -332,4 +334,31 @@ GNSS_API int gnss_load_fields(gnss_ctx* c, const char* dir, int iter) {
     }
     c->eng->set_clock(st, t);
   });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Diagnostics: plane slices (iofuncs.cu:187-327) and iso-surface areas (statistics.cu:1310-1492).
+// ---------------------------------------------------------------------------------------------
+GNSS_API int gnss_get_slice(gnss_ctx* c, int field, int axis, int index, double* out) {
+  if (!out) return GNSS_ERR_ARG;
+  return guard(c, [&] { c->eng->get_slice(field, axis, index, out); });
+}
+
+GNSS_API int gnss_save_slices(gnss_ctx* c, const char* dir, int iter) {
+  if (!dir) return GNSS_ERR_ARG;
+  return guard(c, [&] {
+    gnss_sizes sz;
+    c->eng->sizes(&sz);
+    const long long np = sz.n_physical / (2 * (sz.nzh - 1));
+    std::vector<double> buf((size_t)np);
+    for (int f = 0; f < sz.n_fields; ++f) {
+      c->eng->get_slice(f, 2, (int)(sz.nzh - 1), buf.data());
+      write_raw(std::string(dir) + "/" + field_name(c, f) + "_xy." + std::to_string(iter), buf.data(), np);
+    }
+  });
+}
+
+GNSS_API int gnss_surface_area(gnss_ctx* c, int field, double iso, double* area) {
+  if (!area) return GNSS_ERR_ARG;
+  return guard(c, [&] { *area = c->eng->surface_area(field, iso); });
 }

@@ -75,6 +75,10 @@ class EngineBase {
   virtual void bench_fft(int reps, double* ms_fwd, double* ms_inv) = 0;
   virtual std::string autotune(int reps) = 0;
   virtual void set_tuning(const std::string& spec) = 0;
+  // Diagnostics: one physical plane of a field (or |omega|), and an iso-surface area.
+  virtual void get_slice(int field, int axis, int index, double* out) = 0;
+  virtual double surface_area(int field, double iso) = 0;
+  virtual bool is_colloid(int field) const = 0;
   // Ensemble support: the stream the engine launches on, the kernels of one RK3 step without any
   // host synchronisation, memset or allocation (capturable into a hipGraph), the host-side clock
   // of a replayed step, and the NaN/Inf watchdog.

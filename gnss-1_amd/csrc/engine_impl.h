@@ -414,6 +414,63 @@ class Engine : public EngineBase {
     xc_ = xc;
   }
 
+  // -------------------------------------------------------------------------------------------
+  // Diagnostics. Work buffers ws_ hold the intermediate fields (a complex work buffer holds a
+  // physical field: 2 kzs >= nz); the state, dq and the physical field buffers are untouched.
+  // -------------------------------------------------------------------------------------------
+  bool is_colloid(int field) const override { return colloid_ && field == F_ - 1; }
+
+  // physical values of `field` (or |omega|) in one of ws_[1], ws_[3]
+  T* diag_field(int field) {
+    need_k();
+    if (field < GNSS_FIELD_VORTICITY || field >= F_)
+      throw Error(GNSS_ERR_ARG, "field " + std::to_string(field) + " out of range");
+    T* out;
+    if (field >= 0) {
+      out = reinterpret_cast<T*>(ws_[1]);
+      GNSS_HIP(hipMemcpyAsync(ws_[0], q_[field], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+      inv3d(ws_[0], out, false);
+    } else {
+      CPtrs<T> w{};
+      for (int c = 0; c < 3; ++c) w.p[c] = ws_[c];
+      hipLaunchKernelGGL(k_vorticity<T>, dim3((unsigned)cdiv(nsp_, 256)), dim3(256), 0, s_, g_, ptrs(q_), w, tkx_,
+                         tky_, tkz_, nsp_);
+      for (int c = 0; c < 3; ++c) inv3d(ws_[c], reinterpret_cast<T*>(ws_[3 + c]), false);
+      out = reinterpret_cast<T*>(ws_[3]);
+      hipLaunchKernelGGL(k_magnitude<T>, dim3((unsigned)cdiv(nph_, 256)), dim3(256), 0, s_, out,
+                         reinterpret_cast<const T*>(ws_[4]), reinterpret_cast<const T*>(ws_[5]), nph_);
+    }
+    GNSS_HIP(hipGetLastError());
+    return out;
+  }
+
+  void get_slice(int field, int axis, int index, double* host) override {
+    const int n[3] = {nx_, ny_, nz_};
+    if (axis < 0 || axis > 2 || index < 0 || index >= n[axis])
+      throw Error(GNSS_ERR_ARG, "plane " + std::to_string(index) + " of axis " + std::to_string(axis) + " out of range");
+    const T* a = diag_field(field);
+    const long long np = axis == 2 ? (long long)nx_ * ny_ : (axis == 1 ? (long long)nx_ * nz_ : (long long)ny_ * nz_);
+    double* d = reinterpret_cast<double*>(ws_[2]);  // np doubles <= one complex work buffer
+    hipLaunchKernelGGL(k_plane<T>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, s_, a, d, nx_, ny_, nz_, axis, index);
+    GNSS_HIP(hipGetLastError());
+    GNSS_HIP(hipMemcpyAsync(host, d, sizeof(double) * np, hipMemcpyDeviceToHost, s_));
+    sync();
+  }
+
+  double surface_area(int field, double iso) override {
+    const T* a = diag_field(field);
+    const double dx = p_.lx / nx_, dy = p_.ly / ny_, dz = p_.lz / nz_;
+    timed("k_surface_area", (double)nph_ * sizeof(T), [&] {
+      hipLaunchKernelGGL(k_surface_area<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, a, iso, dx, dy, dz, d_part_);
+    });
+    std::vector<double> part(nx_);
+    GNSS_HIP(hipMemcpyAsync(part.data(), d_part_, sizeof(double) * nx_, hipMemcpyDeviceToHost, s_));
+    sync();
+    double sum = 0.0;
+    for (int x = 0; x < nx_; ++x) sum += part[x];
+    return sum * dx * dy * dz;
+  }
+
   std::string autotune(int reps) override {
     need_k();
     if (reps <= 0) return "{\"key\": \"" + tuning_key() + "\", \"choice\": \"" + tuning_string() + "\"}";

@@ -603,4 +603,89 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Diagnostics
+// ---------------------------------------------------------------------------------------------
+
+// omega^ = i k x u^ over the whole spectral array (calcOmega*Kernel_mgpu, solver.cu:68-114);
+// the kz padding is written as zero.
+template <typename T>
+__global__ void k_vorticity(Geom g, CPtrs<T> q, CPtrs<T> w, const T* __restrict__ kxv,
+                            const T* __restrict__ kyv, const T* __restrict__ kzv, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int iz = (int)(i % g.kzs);
+  const long long r = i / g.kzs;
+  const int iy = (int)(r % g.ny), ix = (int)(r / g.ny);
+  if (iz >= g.nzh) {
+    for (int c = 0; c < 3; ++c) w.p[c][i] = mk<T>(0, 0);
+    return;
+  }
+  const T kx = kxv[ix], ky = kyv[iy], kz = kzv[iz];
+  const cx<T> u = q.p[0][i], v = q.p[1][i], ww = q.p[2][i];
+  w.p[0][i] = csub(ik(ky, ww), ik(kz, v));
+  w.p[1][i] = csub(ik(kz, u), ik(kx, ww));
+  w.p[2][i] = csub(ik(kx, v), ik(ky, u));
+}
+
+// |(a, b, c)| in place over a (VectorMagnitude_kernel, statistics.cu:2005-2018).
+template <typename T>
+__global__ void k_magnitude(T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T x = a[i], y = b[i], z = c[i];
+  a[i] = sqrt(x * x + y * y + z * z);
+}
+
+// One physical plane [x][y][z] -> FP64: axis 2 (z = index) -> [x][y], axis 1 -> [x][z],
+// axis 0 -> [y][z] (writexyfields / writexzfields / writeyzfields, iofuncs.cu:187-260).
+template <typename T>
+__global__ void k_plane(const T* __restrict__ a, double* __restrict__ out, int nx, int ny, int nz, int axis,
+                        int index) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long src;
+  if (axis == 2) {
+    if (i >= (long long)nx * ny) return;
+    src = i * nz + index;
+  } else if (axis == 1) {
+    if (i >= (long long)nx * nz) return;
+    src = ((i / nz) * ny + index) * nz + i % nz;
+  } else {
+    if (i >= (long long)ny * nz) return;
+    src = (long long)index * ny * nz + i;
+  }
+  out[i] = (double)a[src];
+}
+
+// Iso-surface area by the indicator gradient (surfaceArea_kernel_mgpu, statistics.cu:1310-1460):
+// per grid point -(grad F . grad chi) / |grad F| with chi = (F <= iso), central differences on the
+// periodic grid; one block per x plane, fixed per-thread order and tree -> part[x] (FP64).
+template <typename T>
+__global__ void __launch_bounds__(STAT_THREADS)
+k_surface_area(Geom g, const T* __restrict__ F, double iso, double dx, double dy, double dz,
+               double* __restrict__ part) {
+  __shared__ double sh[STAT_THREADS];
+  const int x = blockIdx.x;
+  const int xm = x == 0 ? g.nx - 1 : x - 1, xp = x == g.nx - 1 ? 0 : x + 1;
+  const long long plane = (long long)g.ny * g.nz;
+  double acc[1] = {0.0};
+  for (long long i = threadIdx.x; i < plane; i += STAT_THREADS) {
+    const int y = (int)(i / g.nz), z = (int)(i % g.nz);
+    const int ym = y == 0 ? g.ny - 1 : y - 1, yp = y == g.ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? g.nz - 1 : z - 1, zp = z == g.nz - 1 ? 0 : z + 1;
+    auto at = [&](int a, int b, int c) { return (double)F[((long long)a * g.ny + b) * g.nz + c]; };
+    const double fxm = at(xm, y, z), fxp = at(xp, y, z), fym = at(x, ym, z), fyp = at(x, yp, z);
+    const double fzm = at(x, y, zm), fzp = at(x, y, zp);
+    const double gx = (fxp - fxm) / (2.0 * dx), gy = (fyp - fym) / (2.0 * dy), gz = (fzp - fzm) / (2.0 * dz);
+    const double cxv = ((double)(fxp <= iso) - (double)(fxm <= iso)) / (2.0 * dx);
+    const double cyv = ((double)(fyp <= iso) - (double)(fym <= iso)) / (2.0 * dy);
+    const double czv = ((double)(fzp <= iso) - (double)(fzm <= iso)) / (2.0 * dz);
+    if ((gx == 0 && gy == 0 && gz == 0) || (cxv == 0 && cyv == 0 && czv == 0)) continue;
+    acc[0] += -(gx * cxv + gy * cyv + gz * czv) / sqrt(gx * gx + gy * gy + gz * gz);
+  }
+  double tot[1];
+  block_sum<1>(acc, sh, tot);
+  if (threadIdx.x == 0) part[x] = tot[0];
+}
+
 }  // namespace gnss

@@ -32,6 +32,7 @@ MAX_SCALARS = 4
 FP64, FP32 = 0, 1
 SPECTRAL, PHYSICAL = 0, 1
 FLAG_MFMA_R16, FLAG_UNFUSED, FLAG_COLLOID = 1, 2, 4
+FIELD_VORTICITY = -1
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_ALLOC", 4: "ERR_STATE",
           5: "ERR_NUMERIC", 6: "ERR_UNSUPPORTED"}
@@ -109,6 +110,9 @@ SIGNATURES = {
     "gnss_get_time": (_I, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gnss_save_fields": (_I, [_P, ctypes.c_char_p, _I]),
     "gnss_load_fields": (_I, [_P, ctypes.c_char_p, _I]),
+    "gnss_get_slice": (_I, [_P, _I, _I, _I, _D]),
+    "gnss_save_slices": (_I, [_P, ctypes.c_char_p, _I]),
+    "gnss_surface_area": (_I, [_P, _I, ctypes.c_double, _D]),
     "gnss_timing_enable": (_I, [_P, _I]),
     "gnss_timing_report": (_I, [_P, ctypes.c_char_p, _I]),
     "gnss_bench_fft": (_I, [_P, _I, _D, _D]),
@@ -266,6 +270,28 @@ class Solver:
 
     def load_fields(self, directory: str, it: int):
         self._check(self.lib.gnss_load_fields(self.ctx, directory.encode(), int(it)))
+
+    def slice(self, field: int, axis: int = 2, index: Optional[int] = None) -> np.ndarray:
+        """One physical plane of `field` (FIELD_VORTICITY: |omega|); axis 2 -> [x][y] at
+        z = index (default nz/2), axis 1 -> [x][z], axis 0 -> [y][z]."""
+        n = self.cfg.n
+        if index is None:
+            index = n[axis] // 2 if axis in (0, 1, 2) else 0
+        # an invalid axis gets a one-element buffer: the library rejects it before writing
+        shape = {2: (n[0], n[1]), 1: (n[0], n[2]), 0: (n[1], n[2])}.get(axis, (1,))
+        out = np.empty(shape)
+        self._check(self.lib.gnss_get_slice(self.ctx, int(field), int(axis), int(index), _dptr(out)))
+        return out
+
+    def save_slices(self, directory: str, it: int):
+        """<dir>/<name>_xy.<it>: the z = nz/2 plane of every field in the reference's format."""
+        self._check(self.lib.gnss_save_slices(self.ctx, directory.encode(), int(it)))
+
+    def surface_area(self, field: int, iso: float) -> float:
+        """Area of the iso-surface field = iso (FIELD_VORTICITY: |omega| = iso)."""
+        a = ctypes.c_double()
+        self._check(self.lib.gnss_surface_area(self.ctx, int(field), float(iso), ctypes.byref(a)))
+        return a.value
 
     def time(self):
         t, s = ctypes.c_double(), ctypes.c_int64()

@@ -161,14 +161,34 @@ int gnss_get_time(const gnss_ctx* ctx, double* t, int64_t* step);
 
 /* ≈ save3Dfields / write3Dfields_mgpu (iofuncs.cu:329-432): writes every field in physical space
  *   to <dir>/<name>.<iter> in the reference's raw format — an 8-byte header holding the byte count
- *   as a double, then nx*ny*nz doubles in [x][y][z] order — with names u, v, w, z (scalar 0) and
- *   z1, z2, ... for further scalars, plus <dir>/gnss_state.<iter> ("step <n> t <time>").  The
+ *   as a double, then nx*ny*nz doubles in [x][y][z] order — with names u, v, w, z (scalar 0),
+ *   z1, z2, ... for further scalars and c for the colloid (GNSS_FLAG_COLLOID), plus
+ *   <dir>/gnss_state.<iter> ("step <n> t <time>").  The
  *   RK3 integrator is self-starting, so fields + clock are the whole checkpoint. */
 int gnss_save_fields(gnss_ctx* ctx, const char* dir, int iter);
 /* ≈ loadData / importVelocity / importScalar (iofuncs.cu:462-535): reads the same files (the size
  *   header must equal nx*ny*nz*8, as iofuncs.cu:475-479 checks), forward-transforms and dealiases
  *   them, and restores the clock from gnss_state.<iter> (else step = iter, t = iter * dt). */
 int gnss_load_fields(gnss_ctx* ctx, const char* dir, int iter);
+
+/* Diagnostics (physical space, FP64 results; the state is not changed). */
+#define GNSS_FIELD_VORTICITY (-1)  /* |omega| = |i k x u^| in physical space (VectorMagnitude,
+                                      statistics.cu:2005-2037) */
+/* ≈ writexyfields / writexzfields / writeyzfields (iofuncs.cu:187-260): one physical plane of
+ *   field f (0 <= f < 3 + n_scalars, or GNSS_FIELD_VORTICITY).  axis 2: z = index, out [x][y]
+ *   (nx*ny doubles); axis 1: y = index, out [x][z]; axis 0: x = index, out [y][z]. */
+int gnss_get_slice(gnss_ctx* ctx, int field, int axis, int index, double* out);
+/* ≈ save2Dfields (iofuncs.cu:262-327): writes <dir>/<name>_xy.<iter> for every field — the
+ *   z = nz/2 plane, an 8-byte header holding the byte count as a double, then [x][y] doubles —
+ *   with the names of gnss_save_fields (the colloid is "c", as in the reference). */
+int gnss_save_slices(gnss_ctx* ctx, const char* dir, int iter);
+/* ≈ calcSurfaceArea_mgpu / surfaceArea_kernel_mgpu (statistics.cu:1310-1492; the reference's
+ *   scalar and enstrophy interface areas, statistics.cu:2110, 2132): area of the iso-surface
+ *   F = iso of field f (or GNSS_FIELD_VORTICITY) as  sum over grid points of
+ *   -(grad F . grad chi) / |grad F| dx dy dz,  chi = (F <= iso), with second-order central
+ *   differences on the periodic grid (a point where either gradient vanishes adds nothing).
+ *   The reference's halo exchange between slabs is the periodic wrap here (one GPU). */
+int gnss_surface_area(gnss_ctx* ctx, int field, double iso, double* area);
 
 /* Per-kernel timing with HIP events on the context's stream (engine only; the oracle returns
  *   GNSS_ERR_UNSUPPORTED).  The report is a JSON object: kernel -> {launches, ms, bytes}. */

@@ -1070,9 +1070,12 @@ EXPORT int gnss_get_time(const gnss_ctx* c, double* t, int64_t* step) {
 }
 
 /* Raw field files in the reference's format (iofuncs.cu:329-358 writer, 462-497 reader). */
-static void field_name(int f, char* out, size_t n) {
+/* file names of the reference's field dumps (iofuncs.cu:329-432, 262-327): u v w, z for the
+ * scalar, c for the colloid; further scalars z1, z2, ... */
+static void field_name(const gnss_ctx* c, int f, char* out, size_t n) {
   static const char* vel[3] = {"u", "v", "w"};
   if (f < 3) snprintf(out, n, "%s", vel[f]);
+  else if (c->colloid && f == c->F - 1) snprintf(out, n, "c");
   else if (f == 3) snprintf(out, n, "z");
   else snprintf(out, n, "z%d", f - 3);
 }
@@ -1082,7 +1085,7 @@ EXPORT int gnss_save_fields(gnss_ctx* c, const char* dir, int iter) {
   char nm[16], path[4096];
   for (int f = 0; f < c->F; ++f) {
     c2r3d(c, c->q[f], c->phys[f]);
-    field_name(f, nm, sizeof nm);
+    field_name(c, f, nm, sizeof nm);
     snprintf(path, sizeof path, "%s/%s.%d", dir, nm, iter);
     FILE* fp = fopen(path, "wb");
     if (!fp) return set_err(c, GNSS_ERR_ARG, "cannot open %s", path);
@@ -1104,7 +1107,7 @@ EXPORT int gnss_load_fields(gnss_ctx* c, const char* dir, int iter) {
   if (!c->have_k) return set_err(c, GNSS_ERR_STATE, "call gnss_init_wavenumbers first");
   char nm[16], path[4096];
   for (int f = 0; f < c->F; ++f) {
-    field_name(f, nm, sizeof nm);
+    field_name(c, f, nm, sizeof nm);
     snprintf(path, sizeof path, "%s/%s.%d", dir, nm, iter);
     FILE* fp = fopen(path, "rb");
     if (!fp) return set_err(c, GNSS_ERR_ARG, "cannot open %s", path);
@@ -1130,6 +1133,130 @@ EXPORT int gnss_load_fields(gnss_ctx* c, const char* dir, int iter) {
     if (fscanf(fp, "step %lld t %lf", &a, &b) == 2) { c->step = a; c->t = b; }
     fclose(fp);
   }
+  return GNSS_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Diagnostics: plane slices and iso-surface areas                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Physical values of field f, or of |omega| = |i k x u^| for GNSS_FIELD_VORTICITY
+ * (VectorMagnitude_kernel, statistics.cu:2005-2018), in a buffer of nph doubles. */
+static const double* diag_field(gnss_ctx* c, int f) {
+  if (f >= 0) {
+    c2r3d(c, c->q[f], c->phys[f]);
+    return c->phys[f];
+  }
+  const int nx = c->nx, ny = c->ny, nzh = c->nzh;
+  cplx** w = c->wsp;
+  for (long r = 0; r < (long)nx * ny; ++r) {
+    const int x = (int)(r / ny), y = (int)(r % ny);
+    const double kx = c->kx[x], ky = c->ky[y];
+    for (int k = 0; k < nzh; ++k) {
+      const size_t i = (size_t)r * nzh + k;
+      const double kz = c->kz[k];
+      const cplx u = c->q[0][i], v = c->q[1][i], ww = c->q[2][i];
+      w[0][i] = I * (ky * ww - kz * v);
+      w[1][i] = I * (kz * u - kx * ww);
+      w[2][i] = I * (kx * v - ky * u);
+    }
+  }
+  for (int f2 = 0; f2 < 3; ++f2) c2r3d(c, w[f2], c->wph[f2]);
+  double* m = c->wph[3];
+  for (size_t i = 0; i < c->nph; ++i) {
+    const double a = c->wph[0][i], b = c->wph[1][i], d = c->wph[2][i];
+    m[i] = sqrt(a * a + b * b + d * d);
+  }
+  return m;
+}
+
+static int diag_check(gnss_ctx* c, int field) {
+  if (!c->have_k) return set_err(c, GNSS_ERR_STATE, "call gnss_init_wavenumbers first");
+  if (field < GNSS_FIELD_VORTICITY || field >= c->F)
+    return set_err(c, GNSS_ERR_ARG, "field %d out of range", field);
+  return GNSS_OK;
+}
+
+static void plane_of(const gnss_ctx* c, const double* a, int axis, int index, double* out) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  if (axis == 2) {
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y) out[(size_t)x * ny + y] = a[((size_t)x * ny + y) * nz + index];
+  } else if (axis == 1) {
+    for (int x = 0; x < nx; ++x)
+      for (int z = 0; z < nz; ++z) out[(size_t)x * nz + z] = a[((size_t)x * ny + index) * nz + z];
+  } else {
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) out[(size_t)y * nz + z] = a[((size_t)index * ny + y) * nz + z];
+  }
+}
+
+EXPORT int gnss_get_slice(gnss_ctx* c, int field, int axis, int index, double* out) {
+  if (!c || !out) return GNSS_ERR_ARG;
+  int rc = diag_check(c, field);
+  if (rc) return rc;
+  const int n[3] = {c->nx, c->ny, c->nz};
+  if (axis < 0 || axis > 2 || index < 0 || index >= n[axis])
+    return set_err(c, GNSS_ERR_ARG, "plane %d of axis %d out of range", index, axis);
+  plane_of(c, diag_field(c, field), axis, index, out);
+  return GNSS_OK;
+}
+
+EXPORT int gnss_save_slices(gnss_ctx* c, const char* dir, int iter) {
+  if (!c || !dir) return GNSS_ERR_ARG;
+  if (!c->have_k) return set_err(c, GNSS_ERR_STATE, "call gnss_init_wavenumbers first");
+  const size_t np = (size_t)c->nx * c->ny;
+  double* buf = (double*)malloc(sizeof(double) * np);
+  if (!buf) return set_err(c, GNSS_ERR_ALLOC, "out of host memory");
+  char nm[16], path[4096];
+  for (int f = 0; f < c->F; ++f) {
+    plane_of(c, diag_field(c, f), 2, c->nz / 2, buf);
+    field_name(c, f, nm, sizeof nm);
+    snprintf(path, sizeof path, "%s/%s_xy.%d", dir, nm, iter);
+    FILE* fp = fopen(path, "wb");
+    if (!fp) { free(buf); return set_err(c, GNSS_ERR_ARG, "cannot open %s", path); }
+    const double hdr = (double)np * (double)sizeof(double);
+    int ok = fwrite(&hdr, sizeof(double), 1, fp) == 1 && fwrite(buf, sizeof(double), np, fp) == np;
+    fclose(fp);
+    if (!ok) { free(buf); return set_err(c, GNSS_ERR_ARG, "short write to %s", path); }
+  }
+  free(buf);
+  return GNSS_OK;
+}
+
+/* surfaceArea_kernel_mgpu (statistics.cu:1310-1460): -(grad F . grad chi) / |grad F| summed
+ * over the grid times the cell volume; chi = (F <= iso); central differences, periodic wrap
+ * (the reference's halo exchange, statistics.cu:1177-1209). */
+EXPORT int gnss_surface_area(gnss_ctx* c, int field, double iso, double* area) {
+  if (!c || !area) return GNSS_ERR_ARG;
+  int rc = diag_check(c, field);
+  if (rc) return rc;
+  const double* F = diag_field(c, field);
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  const double dx = c->p.lx / nx, dy = c->p.ly / ny, dz = c->p.lz / nz;
+  double sum = 0.0;
+  for (int x = 0; x < nx; ++x) {
+    const int xm = (x + nx - 1) % nx, xp = (x + 1) % nx;
+    double sx = 0.0;
+    for (int y = 0; y < ny; ++y) {
+      const int ym = (y + ny - 1) % ny, yp = (y + 1) % ny;
+      for (int z = 0; z < nz; ++z) {
+        const int zm = (z + nz - 1) % nz, zp = (z + 1) % nz;
+#define AT(a, b, d) F[((size_t)(a) * ny + (b)) * nz + (d)]
+        const double fxm = AT(xm, y, z), fxp = AT(xp, y, z), fym = AT(x, ym, z), fyp = AT(x, yp, z);
+        const double fzm = AT(x, y, zm), fzp = AT(x, y, zp);
+#undef AT
+        const double gx = (fxp - fxm) / (2.0 * dx), gy = (fyp - fym) / (2.0 * dy), gz = (fzp - fzm) / (2.0 * dz);
+        const double cx = ((fxp <= iso) - (fxm <= iso)) / (2.0 * dx);
+        const double cy = ((fyp <= iso) - (fym <= iso)) / (2.0 * dy);
+        const double cz = ((fzp <= iso) - (fzm <= iso)) / (2.0 * dz);
+        if ((gx == 0 && gy == 0 && gz == 0) || (cx == 0 && cy == 0 && cz == 0)) continue;
+        sx += -(gx * cx + gy * cy + gz * cz) / sqrt(gx * gx + gy * gy + gz * gz);
+      }
+    }
+    sum += sx;
+  }
+  *area = sum * dx * dy * dz;
   return GNSS_OK;
 }
 
