@@ -1,11 +1,14 @@
 // gnss_main.cpp — host driver with the reference's call structure (main.cu:84-252):
-//   parameters -> allocate (gnss_create) -> wavenumbers -> jet IC + forward + dealias ->
-//   stats(0) -> loop { step; stats every n_stats } -> free.
+//   parameters -> allocate (gnss_create) -> wavenumbers -> jet IC + forward + dealias (or a
+//   restart from raw field files) -> stats(0), slices(0) -> loop { step; stats every n_stats;
+//   2D slices every n_vis; checkpoint every n_checkpoint } -> free.
 // The reference hard-codes its configuration in dnsparams.h; here it comes from the command line.
 //
 // usage: gnss_main [--n N | --nx X --ny Y --nz Z] [--re R] [--sc S ...] [--theta T] [--dt DT]
 //                  [--steps K] [--stats-every M] [--kmax K] [--amp A] [--seed S] [--fp32]
 //                  [--l L] [--ly LY] [--config jet64|jet256|jet512|jet1024]
+//                  [--colloid ALPHA [--sc-c SCC] [--theta-c TC]] [--area ISO]
+//                  [--out DIR] [--vis-every V] [--checkpoint-every C] [--restart ITER]
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -13,6 +16,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -25,9 +29,16 @@ static void check(int rc, gnss_ctx* c, const char* what) {
   }
 }
 
-static void print_stats(const gnss_stats_out& s, int S, double ms) {
-  printf("step %6lld  t %9.5f  KE %.10e  eps %.6e  chi0 %.6e  CFL %.4f  div %.2e  (%.3f ms/step)\n",
+static void print_stats(gnss_ctx* ctx, const gnss_stats_out& s, int S, double ms, double iso) {
+  printf("step %6lld  t %9.5f  KE %.10e  eps %.6e  chi0 %.6e  CFL %.4f  div %.2e  (%.3f ms/step)",
          (long long)s.step, s.t, s.ke, s.epsilon, S > 0 ? s.chi[0] : 0.0, s.cfl, s.divergence, ms);
+  if (S > 0 && !std::isnan(iso)) {
+    // scalar interface area (the reference's area_scalar, statistics.cu:2132)
+    double a = 0;
+    check(gnss_surface_area(ctx, 3, iso, &a), ctx, "gnss_surface_area");
+    printf("  area(z=%g) %.6e", iso, a);
+  }
+  printf("\n");
 }
 
 int main(int argc, char** argv) {
@@ -45,7 +56,9 @@ int main(int argc, char** argv) {
   p.seed = 20261202ULL;
   p.dt = 1e-3;
   p.precision = GNSS_FP64;
-  int steps = 100, stats_every = 20;
+  int steps = 100, stats_every = 20, vis_every = 0, ckpt_every = 0, restart = -1;
+  double colloid_alpha = NAN, sc_c = 0.7, iso = NAN;
+  std::string out_dir = ".";
   std::vector<double> sc;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -69,6 +82,14 @@ int main(int argc, char** argv) {
     else if (a == "--amp") p.noise_amp = atof(val());
     else if (a == "--seed") p.seed = strtoull(val(), nullptr, 10);
     else if (a == "--fp32") p.precision = GNSS_FP32;
+    else if (a == "--colloid") colloid_alpha = atof(val());
+    else if (a == "--sc-c") sc_c = atof(val());
+    else if (a == "--theta-c") p.theta_c = atof(val());
+    else if (a == "--area") iso = atof(val());
+    else if (a == "--out") out_dir = val();
+    else if (a == "--vis-every") vis_every = atoi(val());
+    else if (a == "--checkpoint-every") ckpt_every = atoi(val());
+    else if (a == "--restart") restart = atoi(val());
     else if (a == "--config") {
       std::string c = val();
       if (c == "jet64") { p.nx = p.ny = p.nz = 64; p.theta = 0.05; p.re = 1000; sc = {0.7}; p.noise_kmax = 8; p.dt = 1e-3; steps = 100; }
@@ -85,18 +106,31 @@ int main(int argc, char** argv) {
     p.n_scalars = (int)sc.size();
     for (size_t i = 0; i < sc.size() && i < GNSS_MAX_SCALARS; ++i) p.sc[i] = sc[i];
   }
+  if (!std::isnan(colloid_alpha)) {  // the colloid becomes the last scalar (dnsparams.h:28-33)
+    if (p.n_scalars >= GNSS_MAX_SCALARS) { fprintf(stderr, "no room for the colloid scalar\n"); return 2; }
+    p.sc[p.n_scalars++] = sc_c;
+    p.flags |= GNSS_FLAG_COLLOID;
+    p.alpha = colloid_alpha;
+  }
   printf("gnss: grid %dx%dx%d  Re %g  scalars %d  dt %g  steps %d  %s\n", p.nx, p.ny, p.nz, p.re, p.n_scalars,
          p.dt, steps, p.precision == GNSS_FP64 ? "FP64" : "FP32");
 
   gnss_ctx* ctx = nullptr;
   check(gnss_create(&p, &ctx), nullptr, "gnss_create");           // allocate_memory + plans
   check(gnss_init_wavenumbers(ctx), ctx, "gnss_init_wavenumbers"); // initializeWaveNumbers
-  check(gnss_init_jet(ctx), ctx, "gnss_init_jet");                 // initializeJet + forward + deAlias
+  int c0 = 0;
+  if (restart >= 0) {
+    check(gnss_load_fields(ctx, out_dir.c_str(), restart), ctx, "gnss_load_fields");  // loadData
+    c0 = restart;
+  } else {
+    check(gnss_init_jet(ctx), ctx, "gnss_init_jet");  // initializeJet + forward + deAlias
+  }
   gnss_stats_out st;
   check(gnss_stats(ctx, &st, nullptr, nullptr), ctx, "gnss_stats");  // calcTurbStats_mgpu(0)
-  print_stats(st, p.n_scalars, 0.0);
+  print_stats(ctx, st, p.n_scalars, 0.0, iso);
+  if (vis_every > 0) check(gnss_save_slices(ctx, out_dir.c_str(), c0), ctx, "gnss_save_slices");  // save2Dfields(0)
   double total_ms = 0;
-  for (int c = 1; c <= steps; ++c) {
+  for (int c = c0 + 1; c <= c0 + steps; ++c) {
     auto t0 = std::chrono::steady_clock::now();
     check(gnss_step(ctx, 1), ctx, "gnss_step");  // solver_ps
     auto t1 = std::chrono::steady_clock::now();
@@ -104,8 +138,12 @@ int main(int argc, char** argv) {
     total_ms += ms;
     if (stats_every > 0 && c % stats_every == 0) {
       check(gnss_stats(ctx, &st, nullptr, nullptr), ctx, "gnss_stats");
-      print_stats(st, p.n_scalars, ms);
+      print_stats(ctx, st, p.n_scalars, ms, iso);
     }
+    if (vis_every > 0 && c % vis_every == 0)
+      check(gnss_save_slices(ctx, out_dir.c_str(), c), ctx, "gnss_save_slices");  // save2Dfields
+    if (ckpt_every > 0 && c % ckpt_every == 0)
+      check(gnss_save_fields(ctx, out_dir.c_str(), c), ctx, "gnss_save_fields");  // save3Dfields
   }
   printf("total step time %.3f s, %.3f steps/s, %.4e grid-point updates/s\n", total_ms / 1000.0,
          steps / (total_ms / 1000.0), (double)p.nx * p.ny * p.nz * steps / (total_ms / 1000.0));
