@@ -390,11 +390,11 @@ class Engine : public EngineBase {
   }
   std::string tuning_string() const {
     char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d", cx_, cy_, zf_, xf_, cp_, xc_);
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d", cx_, cy_, zf_, xf_, cp_, xc_, zp_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_;
     const char* p = spec.c_str();
     const char* q;
     if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
@@ -403,8 +403,9 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "cy=")) || (q = strstr(p, "\"cy\": "))) cy = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 2) || xf < 0 || xf > 1 ||
-        cp < 0 || cp > 1 || !xchunk_ok(xc))
+        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
@@ -412,6 +413,7 @@ class Engine : public EngineBase {
     xf_ = xf;
     cp_ = cp;
     xc_ = xc;
+    zp_ = zp;
   }
 
   // -------------------------------------------------------------------------------------------
@@ -533,6 +535,24 @@ class Engine : public EngineBase {
         if (ms < best_ms) { best_ms = ms; best = v; }
       }
       cp_ = best;
+    }
+    {
+      // standalone z passes (transforms outside the step: statistics, fields, the 3D FFT):
+      // one block per row group or persistent and pipelined
+      int best = zp_;
+      double best_ms = 1e30;
+      const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
+      for (int v = 0; v <= 1; ++v) {
+        zp_ = v;
+        const double ms = time_it([&] {
+          z_c2r(ws_[0], reinterpret_cast<T*>(ws_[1]), nzh_, scale);
+          z_r2c(reinterpret_cast<const T*>(ws_[1]), ws_[0], kzs_);
+        });
+        snprintf(buf, sizeof buf, ", \"zp=%d\": %.5f", v, ms);
+        rep += buf;
+        if (ms < best_ms) { best_ms = ms; best = v; }
+      }
+      zp_ = best;
     }
     {
       int best = zf_;
@@ -929,9 +949,15 @@ class Engine : public EngineBase {
     int G = std::max(1, 256 / TPC);
     while (G > 1 && (long long)G * zrow_stride(M) * (long long)sizeof(C) > LDS_MAX) --G;
     const long long rows = (long long)nx_ * ny_;
-    hipLaunchKernelGGL((k_r2c_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G),
-                       (size_t)G * zrow_stride(M) * sizeof(C), s_, in, out, tw_.at(M), tw_.at(NZ), rows,
-                       kzs_, out_lim, G);
+    const size_t lds = (size_t)G * zrow_stride(M) * sizeof(C);
+    if (zp_) {
+      const long long groups = cdiv(rows, G);
+      hipLaunchKernelGGL((k_r2c_z_pipe<T, NZ>), dim3((unsigned)std::min(groups, zpipe_blocks(lds, TPC * G))),
+                         dim3(TPC * G), lds, s_, in, out, tw_.at(M), tw_.at(NZ), rows, kzs_, out_lim, G, groups);
+      return;
+    }
+    hipLaunchKernelGGL((k_r2c_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G), lds, s_, in, out,
+                       tw_.at(M), tw_.at(NZ), rows, kzs_, out_lim, G);
   }
   template <int NZ>
   void zc2r_launch(const C* in, T* out, int in_lim, T scale) {
@@ -940,9 +966,23 @@ class Engine : public EngineBase {
     int G = std::max(1, 256 / TPC);
     while (G > 1 && (long long)G * zrow_stride(M) * (long long)sizeof(C) > LDS_MAX) --G;
     const long long rows = (long long)nx_ * ny_;
-    hipLaunchKernelGGL((k_c2r_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G),
-                       (size_t)G * zrow_stride(M) * sizeof(C), s_, in, out, tw_.at(M), tw_.at(NZ), rows,
-                       kzs_, in_lim, scale, G);
+    const size_t lds = (size_t)G * zrow_stride(M) * sizeof(C);
+    if (zp_) {
+      const long long groups = cdiv(rows, G);
+      hipLaunchKernelGGL((k_c2r_z_pipe<T, NZ>), dim3((unsigned)std::min(groups, zpipe_blocks(lds, TPC * G))),
+                         dim3(TPC * G), lds, s_, in, out, tw_.at(M), tw_.at(NZ), rows, kzs_, in_lim, scale, G,
+                         groups);
+      return;
+    }
+    hipLaunchKernelGGL((k_c2r_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G), lds, s_, in, out,
+                       tw_.at(M), tw_.at(NZ), rows, kzs_, in_lim, scale, G);
+  }
+
+  // blocks of a persistent z pass: as many as stay resident on every CU (LDS and wave limits)
+  long long zpipe_blocks(size_t lds, int threads) const {
+    const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / std::max<long long>(1, (long long)lds),
+                                                                       2048 / threads));
+    return per_cu * n_cu_;
   }
 
 #define GNSS_Z_SWITCH(FN, ...)                                                                 \
@@ -1187,7 +1227,7 @@ class Engine : public EngineBase {
   gnss_params p_;
   int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
-  int cp_ = 0, xc_ = 8, n_cu_ = 256;
+  int cp_ = 0, xc_ = 8, zp_ = 0, n_cu_ = 256;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

@@ -301,6 +301,82 @@ k_c2r_z(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __restri
   for (int e = 0; e < Z::E; ++e) dst[t + e * Z::TPC] = v[0][e];
 }
 
+// Persistent, software-pipelined forms of k_r2c_z / k_c2r_z: each block walks row groups
+// blockIdx.x, blockIdx.x + gridDim.x, ... and issues the HBM loads of its next rows before
+// transforming the current ones.  Every row lives in one wavefront (wave-level LDS sync only),
+// so the waves of a block never wait for each other.
+template <typename T, int NZ>
+__global__ void __launch_bounds__(lb_threads((NZ / 2) / Plan<NZ / 2>::E))
+k_r2c_z_pipe(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ twM,
+             const cx<T>* __restrict__ twN, long long nrows, int kzs, int out_lim, int G, long long ngroups) {
+  using Z = ZRow<T, NZ, Plan<NZ / 2>>;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int t = threadIdx.x % Z::TPC;
+  const int g = threadIdx.x / Z::TPC;
+  auto load = [&](long long grp, cx<T>* v) {
+    const long long row = grp * G + g;
+    const bool active = row < nrows;
+    const cx<T>* src = reinterpret_cast<const cx<T>*>(in + (active ? row : 0) * NZ);
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) v[e] = active ? src[t + e * Z::TPC] : mk<T>(0, 0);
+  };
+  long long grp = blockIdx.x;
+  cx<T> v[1][Z::E], nv[Z::E];
+  load(grp, v[0]);
+  while (true) {
+    const long long next = grp + gridDim.x;
+    const bool more = next < ngroups;
+    if (more) load(next, nv);
+    const long long row = grp * G + g;
+    const bool active = row < nrows;
+    cx<T>* dst[1] = {out + (active ? row : 0) * kzs};
+    Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, twM, twN);
+    if (!more) break;
+#pragma unroll
+    for (int e = 0; e < Z::E; ++e) v[0][e] = nv[e];
+    grp = next;
+  }
+}
+
+template <typename T, int NZ>
+__global__ void __launch_bounds__(lb_threads((NZ / 2) / Plan<NZ / 2>::E))
+k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __restrict__ twM,
+             const cx<T>* __restrict__ twN, long long nrows, int kzs, int in_lim, T scale, int G,
+             long long ngroups) {
+  using Z = ZRow<T, NZ, Plan<NZ / 2>>;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int t = threadIdx.x % Z::TPC;
+  const int g = threadIdx.x / Z::TPC;
+  long long grp = blockIdx.x;
+  typename Z::Raw r[1], rn;
+  {
+    const long long row = grp * G + g;
+    Z::load(in + (row < nrows ? row : 0) * kzs, row < nrows, in_lim, t, r[0]);
+  }
+  while (true) {
+    const long long next = grp + gridDim.x;
+    const bool more = next < ngroups;
+    if (more) {
+      const long long row = next * G + g;
+      Z::load(in + (row < nrows ? row : 0) * kzs, row < nrows, in_lim, t, rn);
+    }
+    const long long row = grp * G + g;
+    const bool active = row < nrows;
+    cx<T> v[1][Z::E];
+    Z::template c2r<1>(r, lds, LdsRow{g * Z::RS}, t, twM, twN, scale, v);
+    if (active) {
+      cx<T>* dst = reinterpret_cast<cx<T>*>(out + row * NZ);
+#pragma unroll
+      for (int e = 0; e < Z::E; ++e) dst[t + e * Z::TPC] = v[0][e];
+    }
+    if (!more) break;
+    r[0] = rn;
+    grp = next;
+  }
+}
+
 // Variants of the fused z kernel (chosen by the autotuner): V = 0 the fewest elements per thread
 // with the next rows' loads issued early (FP32), V = 1 the default plan (fewer radix stages, more
 // registers), V = 2 the fewest elements per thread without early loads (fewest registers).
