@@ -12,11 +12,11 @@ def test_table_is_valid():
     assert t, "empty tuning table"
     for key, v in t.items():
         prec, nx, ny, nz, s = key.split()
-        assert prec in ("f32", "f64") and int(s) >= 0
-        assert set(v) == {"cx", "cy", "zf", "xf", "cp"}
+        assert prec in ("f32", "f64") and int(s.rstrip("c")) >= (2 if s.endswith("c") else 0)
+        assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp"}
         allowed = {4, 8} if prec == "f64" else {8, 16}
-        assert v["cx"] in allowed and v["cy"] in allowed
-        assert v["xf"] in (0, 1) and v["cp"] in (0, 1)
+        assert v["cx"] in allowed and v["cy"] in allowed and v.get("xc", v["cx"]) in allowed
+        assert v["xf"] in (0, 1) and v["cp"] in (0, 1) and v.get("zp", 0) in (0, 1)
         assert v["zf"] in ((0,) if prec == "f64" else (0, 1, 2))
     # one entry per line: the engine scans the file line by line for its key
     lines = [l for l in open(TABLE).read().splitlines() if l.startswith('"')]
@@ -26,5 +26,6 @@ def test_table_is_valid():
 def test_measured_workloads_are_tuned():
     t = json.load(open(TABLE))
     for key in ["f32 512 512 512 1", "f64 512 512 512 1", "f64 256 256 256 2", "f32 1024 1024 1024 1",
-                "f32 64 64 64 1", "f32 96 96 96 1", "f32 128 128 128 1"]:
+                "f32 64 64 64 1", "f32 96 96 96 1", "f32 128 128 128 1", "f32 512 512 512 2c",
+                "f64 512 512 512 2c"]:
         assert key in t, key
