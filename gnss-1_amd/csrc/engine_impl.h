@@ -404,7 +404,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || zf < 0 || zf > (sizeof(T) == 8 ? 0 : 2) || xf < 0 || xf > 1 ||
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 1 ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
@@ -557,7 +557,8 @@ class Engine : public EngineBase {
     {
       int best = zf_;
       double best_ms = 1e30;
-      for (int v = 0; v <= (sizeof(T) == 8 ? 0 : 2); ++v) {
+      for (int v = 0; v <= 4; ++v) {
+        if (!zvariant_ok(v)) continue;
         zf_ = v;
         const double ms = time_it([&] { z_fused(); });
         snprintf(buf, sizeof buf, ", \"zf=%d\": %.5f", v, ms);
@@ -616,6 +617,8 @@ class Engine : public EngineBase {
   }
 
  private:
+  // fused z variants built: FP32 0-4, FP64 0 and 3
+  static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 4) : (v == 0 || v == 3); }
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
   bool xchunk_ok(int c) const {
     return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(nx_, c);
@@ -1061,7 +1064,9 @@ class Engine : public EngineBase {
     if constexpr (sizeof(T) == 4) {
       if (zf_ == 1) return zfused_launch_v<NZ, 1>(a0, scale);
       if (zf_ == 2) return zfused_launch_v<NZ, 2>(a0, scale);
+      if (zf_ == 4) return zfused_launch_v<NZ, 4>(a0, scale);
     }
+    if (zf_ == 3) return zfused_launch_v<NZ, 3>(a0, scale);
     zfused_launch_v<NZ, 0>(a0, scale);
   }
 
@@ -1071,7 +1076,7 @@ class Engine : public EngineBase {
     using P = typename FusedPlan<T, M, V>::type;
     constexpr int TPC = M / P::E;
     int G = std::max(1, 256 / TPC);
-    const long long row_lds = (long long)FUSED_B * zrow_stride(M) * (long long)sizeof(C);
+    const long long row_lds = (long long)FUSED_B * FusedPlan<T, M, V>::pad::stride(M) * (long long)sizeof(C);
     while (G > 1 && G * row_lds > LDS_MAX) --G;
     ZFusedArgs<T> a = a0;
     a.G = G;

@@ -169,10 +169,15 @@ struct LdsCol {
   int base, C;
   __device__ __forceinline__ int operator()(int pos) const { return base + pos * C; }
 };
-struct LdsRow {
+// Row tiles are padded by A entries every 2^S positions: with S >= 4 the padding of a radix-4/8
+// stage's accesses separates into a per-thread base plus compile-time offsets (no address VALU).
+template <int S, int A> struct LdsRowP {
   int base;
-  __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+  __device__ __forceinline__ int operator()(int pos) const { return base + pos + A * (pos >> S); }
+  // entries one row of M positions (plus the Nyquist slot) occupies
+  __host__ __device__ static constexpr int stride(int M) { return M + 2 + A * (M >> S); }
 };
+using LdsRow = LdsRowP<4, 1>;
 
 // Synchronisation between radix stages: a whole block, or only the calling wavefront when every
 // transform lives inside one wavefront (its LDS traffic is then ordered by the wave itself and

@@ -142,12 +142,12 @@ __host__ __device__ constexpr int zrow_stride(int M) { return M + 2 + (M >> 4); 
 // values is packed as M = NZ/2 complex pairs (z = 2n, 2n+1); the half-length transform is
 // unpacked with exp(-2 pi i k / NZ) (R2C) or packed the same way before it (C2R).  TPC divides
 // 64, so a row never straddles wavefronts and all row-local LDS traffic is wave-synchronised.
-template <typename T, int NZ, typename P>
+template <typename T, int NZ, typename P, typename Pad = LdsRow>
 struct ZRow {
   static constexpr int M = NZ / 2;
   static constexpr int E = P::E;
   static constexpr int TPC = M / E;
-  static constexpr int RS = zrow_stride(M);
+  static constexpr int RS = Pad::stride(M);
   static_assert(64 % TPC == 0, "a z row must lie inside one wavefront");
 
   // Registers of one spectral row as loaded from HBM (entries k >= lim read as 0).
@@ -180,7 +180,7 @@ struct ZRow {
   // The pairing of k with M - k is a lane shuffle; only the radix stages use LDS, and the B rows
   // go through each stage together (LDS region of row b at idx + b * RS).
   template <int B>
-  __device__ __forceinline__ static void c2r(const Raw* r, cx<T>* lds, const LdsRow& idx, int t,
+  __device__ __forceinline__ static void c2r(const Raw* r, cx<T>* lds, const Pad& idx, int t,
                                              const cx<T>* __restrict__ twM,
                                              const cx<T>* __restrict__ twN, T scale, cx<T> (*v)[E]) {
 #pragma unroll
@@ -210,7 +210,7 @@ struct ZRow {
       }
     }
     lds_sync<true>();
-    fft_batch<T, M, +1, LdsRow, P, true, B, RS>(v, lds, t, idx, twM);
+    fft_batch<T, M, +1, Pad, P, true, B, RS>(v, lds, t, idx, twM);
 #pragma unroll
     for (int b = 0; b < B; ++b)
 #pragma unroll
@@ -221,11 +221,11 @@ struct ZRow {
   // (the Nyquist value at k = M and zero padding up to kzs included).
   template <int B>
   __device__ __forceinline__ static void r2c(cx<T> (*v)[E], cx<T>* const* dst, bool active, int lim,
-                                             int kzs, cx<T>* lds, const LdsRow& idx, int t,
+                                             int kzs, cx<T>* lds, const Pad& idx, int t,
                                              const cx<T>* __restrict__ twM,
                                              const cx<T>* __restrict__ twN) {
     lds_sync<true>();
-    fft_batch<T, M, -1, LdsRow, P, true, B, RS>(v, lds, t, idx, twM);
+    fft_batch<T, M, -1, Pad, P, true, B, RS>(v, lds, t, idx, twM);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       cx<T> zc[E];
@@ -379,18 +379,33 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
 
 // Variants of the fused z kernel (chosen by the autotuner): V = 0 the fewest elements per thread
 // with the next rows' loads issued early (FP32), V = 1 the default plan (fewer radix stages, more
-// registers), V = 2 the fewest elements per thread without early loads (fewest registers).
+// registers), V = 2 the fewest elements per thread without early loads (fewest registers),
+// V = 3 as 2 with a wider LDS row padding (fewer bank conflicts in the radix-4 exchanges: 5 entries
+// every 32 for 8-byte, 3 every 16 for 16-byte elements), V = 4 the default plan without early loads.
 template <typename T, int M, int V> struct FusedPlan {
   using type = PlanLow<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
+  using pad = LdsRow;
 };
 template <typename T, int M> struct FusedPlan<T, M, 1> {
   using type = Plan<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
+  using pad = LdsRow;
 };
 template <typename T, int M> struct FusedPlan<T, M, 2> {
   using type = PlanLow<M>;
   static constexpr bool prefetch = false;
+  using pad = LdsRow;
+};
+template <typename T, int M> struct FusedPlan<T, M, 3> {
+  using type = PlanLow<M>;
+  static constexpr bool prefetch = false;
+  using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<4, 3>>::type;
+};
+template <typename T, int M> struct FusedPlan<T, M, 4> {
+  using type = Plan<M>;
+  static constexpr bool prefetch = false;
+  using pad = LdsRow;
 };
 
 // Work rows: u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar, and
@@ -433,9 +448,10 @@ constexpr int FUSED_B = 3;
 template <typename T, int NZ, int V>
 __global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E))
 k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, T scale) {
-  using Z = ZRow<T, NZ, typename FusedPlan<T, NZ / 2, V>::type>;
+  using FP = FusedPlan<T, NZ / 2, V>;
+  using Z = ZRow<T, NZ, typename FP::type, typename FP::pad>;
   constexpr int E = Z::E;
-  constexpr bool PREFETCH = FusedPlan<T, NZ / 2, V>::prefetch;
+  constexpr bool PREFETCH = FP::prefetch;
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int t = threadIdx.x % Z::TPC;
@@ -443,7 +459,7 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
   const long long row = (long long)blockIdx.x * a.G + g;
   const bool active = row < a.nrows;
   const long long off = (active ? row : 0) * a.kzs;
-  const LdsRow idx{g * Z::RS * FUSED_B};
+  const typename FP::pad idx{g * Z::RS * FUSED_B};
   typename Z::Raw r0[3], r1[3];
   cx<T> vel[3][E], om[3][E];
 #pragma unroll
