@@ -168,16 +168,23 @@ def test_mfma_radix16_transform(g, engine, n):
     print(f"\n[mfma fft {n}]", flush=True)
     x = np.random.default_rng(11).standard_normal(n)
     ref = np.fft.rfftn(x).transpose(1, 0, 2)
-    out = {}
+    out, names = {}, {}
     for flags in (0, g.FLAG_MFMA_R16):
         a = g.Solver(g.Config(n=n, sc=(), precision=g.FP32, flags=flags), engine)
+        a.timing(True)
         a.set_physical(0, x)
         a.forward(0)
         out[flags] = a.get_spectral(0)
         a.inverse(0)
         assert rel(a.get_physical(0), x) < 3e-6
+        names[flags] = set(a.timing_report())
     assert rel(out[g.FLAG_MFMA_R16], ref) < 3e-6
     assert rel(out[g.FLAG_MFMA_R16], out[0]) < 3e-6
+    # the MFMA stage really ran: its passes are timed under their own names and its split-FP16
+    # products round differently from the VALU butterflies
+    assert {"fft_x_fwd_mfma", "fft_x_inv_mfma"} <= names[g.FLAG_MFMA_R16], names
+    assert not any("mfma" in k for k in names[0])
+    assert out[g.FLAG_MFMA_R16].tobytes() != out[0].tobytes()
 
 
 def test_mfma_step_matches_valu(g, engine):
