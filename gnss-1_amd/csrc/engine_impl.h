@@ -404,7 +404,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 1 ||
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 2 ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
@@ -584,13 +584,13 @@ class Engine : public EngineBase {
         }
         int best = xf_, best_xc = xc_;
         double best_ms = 1e30;
-        for (int v = 0; v <= 1; ++v) {
+        for (int v = 0; v <= 2; ++v) {
           for (int c : {4, 8, 16}) {
             if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once
             xf_ = v;
             if (v) xc_ = c;
             const double ms = time_it([&] { substage(1); });
-            if (v) snprintf(buf, sizeof buf, ", \"xf=1 xc=%d\": %.5f", c, ms);
+            if (v) snprintf(buf, sizeof buf, ", \"xf=%d xc=%d\": %.5f", v, c, ms);
             else snprintf(buf, sizeof buf, ", \"xf=0\": %.5f", ms);
             rep += buf;
             if (ms < best_ms) { best_ms = ms; best = v; best_xc = xc_; }
@@ -1086,7 +1086,88 @@ class Engine : public EngineBase {
 
   // x passes fused with the spectral ends of a substage, one launch per field group
   // (velocity, then each scalar).
-  bool fused_x_ok() const { return xf_ && !(p_.flags & GNSS_FLAG_MFMA_R16); }
+  // x-pass mode of a substage: 0 prologue kernel + separate x passes; 1 x passes fused with the
+  // prologue and the RK3 update per field group; 2 per-field x inverse passes that form their
+  // work field on the fly (forward side as in mode 0).  The MFMA radix-16 path uses mode 0.
+  int x_mode() const { return (p_.flags & GNSS_FLAG_MFMA_R16) ? 0 : xf_; }
+
+  // Work field w of a substage as an operation on one or two state fields.
+  XOp xop_of(int w, const C*& A, const C*& B) const {
+    B = nullptr;
+    if (w < 3) { A = q_[w]; return XOp{0, 0, 0}; }
+    if (w == 3) { A = q_[2]; B = q_[1]; return XOp{2, 1, 2}; }  // omega_x = i (ky w - kz v)
+    if (w == 4) { A = q_[0]; B = q_[2]; return XOp{2, 2, 0}; }  // omega_y = i (kz u - kx w)
+    if (w == 5) { A = q_[1]; B = q_[0]; return XOp{2, 0, 1}; }  // omega_z = i (kx v - ky u)
+    if (w < 6 + 3 * S_) {                                       // grad phi_s
+      A = q_[3 + (w - 6) / 3];
+      return XOp{1, (w - 6) % 3, 0};
+    }
+    if (w == 6 + 3 * S_) { A = q_[2 + S_]; return XOp{0, 0, 0}; }  // colloid c
+    A = q_[3];
+    return XOp{3, 0, 0};                                            // lap Z
+  }
+
+  template <int L, int CC>
+  void xop_launch(int w, ColArgs a) {
+    constexpr int TPC = L / Plan<L>::E;
+    int G = std::max(1, 256 / (CC * TPC));
+    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
+    a.G = G;
+    const int threads = CC * TPC * G;
+    const size_t lds = (size_t)G * L * CC * sizeof(C);
+    const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
+    const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds, 2048 / threads));
+    const long long grid = std::min<long long>(blocks, per_cu * n_cu_);
+    const C* A;
+    const C* B;
+    const XOp op = xop_of(w, A, B);
+    hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, ws_[w], A,
+                       B ? B : A, op, tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks);
+  }
+  template <int L>
+  void xop_c(int w, const ColArgs& a) {
+    if constexpr (sizeof(T) == 8) {
+      if (xc_ == 8) xop_launch<L, 8>(w, a);
+      else xop_launch<L, 4>(w, a);
+    } else {
+      if (xc_ == 16) xop_launch<L, 16>(w, a);
+      else xop_launch<L, 8>(w, a);
+    }
+  }
+  // x inverse pass of work field w formed from the state (mode 2)
+  void x_inv_op(int w) {
+    ColArgs a;
+    a.estride = (long long)ny_ * kzs_;
+    a.ostride = kzs_;
+    a.outer_keep = g_.keep_y;
+    a.outer_n = ny_;
+    a.nouter = 2 * g_.keep_y + 1;
+    a.nchunks = kz_lim_ / xc_;
+    a.in_keep = g_.keep_x;
+    a.out_keep = -1;
+    a.G = 1;
+    const C* A;
+    const C* B;
+    const int nin = xop_of(w, A, B).kind == 2 ? 2 : 1;
+    const double bytes = (double)a.nouter * a.nchunks * xc_ * (nin * (2.0 * g_.keep_x + 1) + nx_) * sizeof(C);
+    timed("fft_x_inv_op", bytes, [&] {
+      switch (nx_) {
+        case 16: xop_c<16>(w, a); break;
+        case 24: xop_c<24>(w, a); break;
+        case 32: xop_c<32>(w, a); break;
+        case 48: xop_c<48>(w, a); break;
+        case 64: xop_c<64>(w, a); break;
+        case 96: xop_c<96>(w, a); break;
+        case 128: xop_c<128>(w, a); break;
+        case 192: xop_c<192>(w, a); break;
+        case 256: xop_c<256>(w, a); break;
+        case 384: xop_c<384>(w, a); break;
+        case 512: xop_c<512>(w, a); break;
+        case 1024: xop_c<1024>(w, a); break;
+        default: throw Error(GNSS_ERR_UNSUPPORTED, "x pass length " + std::to_string(nx_));
+      }
+    });
+  }
 
   template <int L, int CC, int GRP>
   void xgrp_launch(bool pro, int G, int nchunks, int scalar, const RkCoef& rc) {
@@ -1187,11 +1268,12 @@ class Engine : public EngineBase {
     const Box b = kept_box();
     const long long nb = (long long)b.bx * b.by * b.bz;
     const RkCoef c = rk_coef(s);
-    const bool xf = !unfused && fused_x_ok();
+    const int xm = unfused ? 0 : x_mode();
+    const bool xf = xm == 1;
     if (xf) {
       x_fused(true, -1, c);
       for (int q = 0; q < S_; ++q) x_fused(true, q, c);
-    } else {
+    } else if (xm == 0) {
       timed("k_prologue", (double)nb * (F_ + W_) * sizeof(C), [&] {
         hipLaunchKernelGGL(k_prologue<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_),
                            ptrs(ws_), tkx_, tky_, tkz_, S_, colloid_, nb);
@@ -1207,7 +1289,8 @@ class Engine : public EngineBase {
       for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
     } else {
       for (int w = 0; w < W_; ++w) {
-        if (!xf) col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+        if (xm == 2) x_inv_op(w);
+        else if (xm == 0) col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
         col_pass(ws_[w], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
       }
       z_fused();

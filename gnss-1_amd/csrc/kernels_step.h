@@ -511,6 +511,93 @@ k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, cons
   }
 }
 
+// One work field of a substage formed from one or two state fields (the spectral prologue,
+// calcOmega*Kernel_mgpu solver.cu:68-114, gradientKernel_mgpu solver.cu:233-256, the colloid's
+// c and lap Z, solver.cu:419-437):  kind 0: A;  1: i k_da A;  2: i (k_da A - k_db B);  3: -k^2 A.
+// Axis d picks kx (per row of the x column), ky (per tile) or kz (per lane).
+struct XOp {
+  int kind, da, db;
+};
+
+// Inverse x pass of one work field with its prologue folded into the loads: persistent and
+// software-pipelined like k_col_pipe (the next tile's inputs are loaded before the current tile
+// transforms).  Inputs and output share the spectral layout, so a column of A (and B) sits at the
+// output column's offset; only kept kx rows are read, every row is written.
+template <typename T, int L, int C>
+__global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
+k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, XOp op,
+               const T* __restrict__ kxv, const T* __restrict__ kyv, const T* __restrict__ kzv,
+               const cx<T>* __restrict__ tw, ColArgs a, int ntiles) {
+  constexpr int E = Plan<L>::E;
+  constexpr int TPC = L / E;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = (tid / C) % TPC;
+  const int g = tid / (C * TPC);
+  const bool two = op.kind == 2;
+  auto offset_of = [&](int tile, bool& active, int& iy, int& iz) {
+    const int chunk = tile % a.nchunks;
+    const int o = (tile / a.nchunks) * a.G + g;
+    active = o < a.nouter;
+    iy = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
+    iz = chunk * C + c;
+    return (long long)iy * a.ostride + iz;
+  };
+  auto load = [&](int tile, cx<T>* va, cx<T>* vb) {
+    bool active;
+    int iy, iz;
+    const long long off = offset_of(tile, active, iy, iz);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int p = t + e * TPC;
+      const bool ld = active && row_kept(p, L, a.in_keep);
+      va[e] = ld ? A[off + (long long)p * a.estride] : mk<T>(0, 0);
+      vb[e] = (ld && two) ? B[off + (long long)p * a.estride] : mk<T>(0, 0);
+    }
+  };
+  auto form = [&](int tile, const cx<T>* va, const cx<T>* vb, cx<T>* v) {
+    bool active;
+    int iy, iz;
+    (void)offset_of(tile, active, iy, iz);
+    const T ky = kyv[iy], kz = kzv[iz];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const T kx = kxv[t + e * TPC];
+      const T ka = op.da == 0 ? kx : (op.da == 1 ? ky : kz);
+      const T kb = op.db == 0 ? kx : (op.db == 1 ? ky : kz);
+      if (op.kind == 0) v[e] = va[e];
+      else if (op.kind == 1) v[e] = ik(ka, va[e]);
+      else if (op.kind == 2) v[e] = csub(ik(ka, va[e]), ik(kb, vb[e]));
+      else {
+        const T k2 = kx * kx + ky * ky + kz * kz;
+        v[e] = mk<T>(-k2 * va[e].x, -k2 * va[e].y);
+      }
+    }
+  };
+  int tile = blockIdx.x;
+  cx<T> va[E], vb[E], v[E];
+  load(tile, va, vb);
+  const LdsCol idx{g * L * C + c, C};
+  while (true) {
+    form(tile, va, vb, v);
+    const int next = tile + gridDim.x;
+    const bool more = next < ntiles;
+    if (more) load(next, va, vb);
+    fft<T, L, +1>(v, lds, t, idx, tw);
+    bool active;
+    int iy, iz;
+    const long long off = offset_of(tile, active, iy, iz);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) out[off + (long long)(t + e * TPC) * a.estride] = v[e];
+    }
+    if (!more) break;
+    tile = next;
+  }
+}
+
 // Forward x pass of one field group's nonlinear terms fused with the projection (velocity),
 // the sign of the scalar term (computeRHSKernel_mgpu solver.cu:461-521), the mask and the RK3-IF
 // update of the state (replaces solver.cu:547-657): the transformed nonlinear terms never return
