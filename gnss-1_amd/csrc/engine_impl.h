@@ -404,7 +404,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 2 ||
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 3 ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
@@ -584,7 +584,7 @@ class Engine : public EngineBase {
         }
         int best = xf_, best_xc = xc_;
         double best_ms = 1e30;
-        for (int v = 0; v <= 2; ++v) {
+        for (int v = 0; v <= 3; ++v) {
           for (int c : {4, 8, 16}) {
             if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once
             xf_ = v;
@@ -1088,7 +1088,8 @@ class Engine : public EngineBase {
   // (velocity, then each scalar).
   // x-pass mode of a substage: 0 prologue kernel + separate x passes; 1 x passes fused with the
   // prologue and the RK3 update per field group; 2 per-field x inverse passes that form their
-  // work field on the fly (forward side as in mode 0).  The MFMA radix-16 path uses mode 0.
+  // work field on the fly (forward side as in mode 0); 3 the inverse side of mode 2 with the
+  // forward side of mode 1.  The MFMA radix-16 path uses mode 0.
   int x_mode() const { return (p_.flags & GNSS_FLAG_MFMA_R16) ? 0 : xf_; }
 
   // Work field w of a substage as an operation on one or two state fields.
@@ -1269,7 +1270,8 @@ class Engine : public EngineBase {
     const long long nb = (long long)b.bx * b.by * b.bz;
     const RkCoef c = rk_coef(s);
     const int xm = unfused ? 0 : x_mode();
-    const bool xf = xm == 1;
+    const bool xf = xm == 1;                  // group-fused inverse side
+    const bool xfwd = xm == 1 || xm == 3;     // group-fused forward side
     if (xf) {
       x_fused(true, -1, c);
       for (int q = 0; q < S_; ++q) x_fused(true, q, c);
@@ -1289,17 +1291,17 @@ class Engine : public EngineBase {
       for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
     } else {
       for (int w = 0; w < W_; ++w) {
-        if (xm == 2) x_inv_op(w);
+        if (xm >= 2) x_inv_op(w);
         else if (xm == 0) col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
         col_pass(ws_[w], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
       }
       z_fused();
       for (int f = 0; f < F_; ++f) {
         col_pass(nl_[f], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
-        if (!xf) col_pass(nl_[f], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
+        if (!xfwd) col_pass(nl_[f], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
       }
     }
-    if (xf) {
+    if (xfwd) {
       x_fused(false, -1, c);
       for (int q = 0; q < S_; ++q) x_fused(false, q, c);
       return;

@@ -148,19 +148,20 @@ def test_fused_matches_unfused(g, engine, prec):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_per_field_x_inverse_bit_identical(g, engine, prec):
-    """x mode 2 forms each work field inside its x inverse pass (u, omega, grad phi, the colloid's
-    c and lap Z): the same operations as the prologue kernel followed by the x pass, so the step
-    agrees with mode 0 to the last bit."""
+    """x modes 2 and 3 form each work field inside its x inverse pass (u, omega, grad phi, the
+    colloid's c and lap Z): the same operations as the prologue (mode 0) or the group-fused
+    inverse (mode 1) followed by the transform, so the steps agree to the last bit."""
     cfg = g.with_colloid(g.Config(n=(64, 48, 96), re=400.0, sc=(0.7, 2.0), theta=0.08, noise_kmax=8.0,
                                   noise_amp=0.2, dt=1e-3, precision=prec), alpha=0.02)
-    out = []
-    for spec in ("xf=0", "xf=2"):
-        a = g.Solver(cfg, engine).setup()
-        a.set_tuning(spec)
-        a.step(3)
-        out.append([a.get_spectral(f) for f in range(6)])
-    for f in range(6):
-        assert out[0][f].tobytes() == out[1][f].tobytes(), f
+    for pair in (("xf=0", "xf=2"), ("xf=1", "xf=3")):
+        out = []
+        for spec in pair:
+            a = g.Solver(cfg, engine).setup()
+            a.set_tuning(spec)
+            a.step(3)
+            out.append([a.get_spectral(f) for f in range(6)])
+        for f in range(6):
+            assert out[0][f].tobytes() == out[1][f].tobytes(), (pair, f)
 
 
 def test_fp32_error_vs_fp64(g, engine):
