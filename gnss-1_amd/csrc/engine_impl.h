@@ -1122,8 +1122,12 @@ class Engine : public EngineBase {
     const C* A;
     const C* B;
     const XOp op = xop_of(w, A, B);
-    hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, ws_[w], A,
-                       B ? B : A, op, tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks);
+    if (op.kind == 2)
+      hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC, true>), dim3((unsigned)grid), dim3(threads), lds, s_, ws_[w],
+                         A, B, op, tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks);
+    else
+      hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC, false>), dim3((unsigned)grid), dim3(threads), lds, s_, ws_[w],
+                         A, A, op, tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks);
   }
   template <int L>
   void xop_c(int w, const ColArgs& a) {

@@ -521,9 +521,9 @@ struct XOp {
 
 // Inverse x pass of one work field with its prologue folded into the loads: persistent and
 // software-pipelined like k_col_pipe (the next tile's inputs are loaded before the current tile
-// transforms).  Inputs and output share the spectral layout, so a column of A (and B) sits at the
+// transforms).  TWO: the two-input operation (kind 2); otherwise only A is read.  Inputs and output share the spectral layout, so a column of A (and B) sits at the
 // output column's offset; only kept kx rows are read, every row is written.
-template <typename T, int L, int C>
+template <typename T, int L, int C, bool TWO>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
 k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, XOp op,
                const T* __restrict__ kxv, const T* __restrict__ kyv, const T* __restrict__ kzv,
@@ -536,7 +536,6 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
   const int c = tid % C;
   const int t = (tid / C) % TPC;
   const int g = tid / (C * TPC);
-  const bool two = op.kind == 2;
   auto offset_of = [&](int tile, bool& active, int& iy, int& iz) {
     const int chunk = tile % a.nchunks;
     const int o = (tile / a.nchunks) * a.G + g;
@@ -545,7 +544,9 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
     iz = chunk * C + c;
     return (long long)iy * a.ostride + iz;
   };
-  auto load = [&](int tile, cx<T>* va, cx<T>* vb) {
+  // A is loaded one tile ahead; B (two-input operation) when the tile is formed, which keeps the
+  // register footprint of a 16-column tile without spills
+  auto load_col = [&](const cx<T>* __restrict__ src, int tile, cx<T>* va) {
     bool active;
     int iy, iz;
     const long long off = offset_of(tile, active, iy, iz);
@@ -553,23 +554,24 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
     for (int e = 0; e < E; ++e) {
       const int p = t + e * TPC;
       const bool ld = active && row_kept(p, L, a.in_keep);
-      va[e] = ld ? A[off + (long long)p * a.estride] : mk<T>(0, 0);
-      vb[e] = (ld && two) ? B[off + (long long)p * a.estride] : mk<T>(0, 0);
+      va[e] = ld ? src[off + (long long)p * a.estride] : mk<T>(0, 0);
     }
   };
-  auto form = [&](int tile, const cx<T>* va, const cx<T>* vb, cx<T>* v) {
+  auto form = [&](int tile, const cx<T>* va, cx<T>* v) {
     bool active;
     int iy, iz;
     (void)offset_of(tile, active, iy, iz);
     const T ky = kyv[iy], kz = kzv[iz];
+    cx<T> vb[TWO ? E : 1];
+    if (TWO) load_col(B, tile, vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const T kx = kxv[t + e * TPC];
       const T ka = op.da == 0 ? kx : (op.da == 1 ? ky : kz);
       const T kb = op.db == 0 ? kx : (op.db == 1 ? ky : kz);
-      if (op.kind == 0) v[e] = va[e];
+      if (TWO) v[e] = csub(ik(ka, va[e]), ik(kb, vb[TWO ? e : 0]));
+      else if (op.kind == 0) v[e] = va[e];
       else if (op.kind == 1) v[e] = ik(ka, va[e]);
-      else if (op.kind == 2) v[e] = csub(ik(ka, va[e]), ik(kb, vb[e]));
       else {
         const T k2 = kx * kx + ky * ky + kz * kz;
         v[e] = mk<T>(-k2 * va[e].x, -k2 * va[e].y);
@@ -577,14 +579,14 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
     }
   };
   int tile = blockIdx.x;
-  cx<T> va[E], vb[E], v[E];
-  load(tile, va, vb);
+  cx<T> va[E], v[E];
+  load_col(A, tile, va);
   const LdsCol idx{g * L * C + c, C};
   while (true) {
-    form(tile, va, vb, v);
+    form(tile, va, v);
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
-    if (more) load(next, va, vb);
+    if (more) load_col(A, next, va);
     fft<T, L, +1>(v, lds, t, idx, tw);
     bool active;
     int iy, iz;

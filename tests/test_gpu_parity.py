@@ -147,10 +147,11 @@ def test_fused_matches_unfused(g, engine, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_per_field_x_inverse_bit_identical(g, engine, prec):
+def test_per_field_x_inverse_matches(g, engine, prec):
     """x modes 2 and 3 form each work field inside its x inverse pass (u, omega, grad phi, the
-    colloid's c and lap Z): the same operations as the prologue (mode 0) or the group-fused
-    inverse (mode 1) followed by the transform, so the steps agree to the last bit."""
+    colloid's c and lap Z) with the operations of the prologue (mode 0) or of the group-fused
+    inverse (mode 1) before the same transform; the compiler may contract a product and a sum into
+    one FMA differently in the two kernels, so agreement is to rounding (FP64 1e-13, FP32 2e-6)."""
     cfg = g.with_colloid(g.Config(n=(64, 48, 96), re=400.0, sc=(0.7, 2.0), theta=0.08, noise_kmax=8.0,
                                   noise_amp=0.2, dt=1e-3, precision=prec), alpha=0.02)
     for pair in (("xf=0", "xf=2"), ("xf=1", "xf=3")):
@@ -161,7 +162,7 @@ def test_per_field_x_inverse_bit_identical(g, engine, prec):
             a.step(3)
             out.append([a.get_spectral(f) for f in range(6)])
         for f in range(6):
-            assert out[0][f].tobytes() == out[1][f].tobytes(), (pair, f)
+            assert rel(out[1][f], out[0][f]) <= (1e-13 if prec == 0 else 2e-6), (pair, f)
 
 
 def test_fp32_error_vs_fp64(g, engine):
