@@ -46,9 +46,11 @@ class Engine : public EngineBase {
     cy_ = col_chunk(ny_);
     xc_ = cx_;
     zf_ = 0;
-    // kz padding and the pruned kz limit use the widest column tile any variant may choose
+    load_tuning();  // may set kp, which fixes the row stride below
+    // kz padding and the pruned kz limit use the widest column tile any variant may choose; kp
+    // extra tiles of padding move the x pass's row stride (ny kzs elements) off large powers of two
     const int cmax = sizeof(T) == 8 ? 8 : 16;
-    kzs_ = round_up(nzh_, cmax);
+    kzs_ = round_up(nzh_, cmax) + kp_ * cmax;
     g_.nx = nx_; g_.ny = ny_; g_.nz = nz_; g_.nzh = nzh_; g_.kzs = kzs_;
     g_.keep_x = nx_ / 3; g_.keep_y = ny_ / 3; g_.keep_z = nz_ / 3;
     int64_t ax, ay, az, g2;
@@ -57,7 +59,6 @@ class Engine : public EngineBase {
     kz_lim_ = std::min(round_up(g_.keep_z + 1, cmax), kzs_);
     nsp_ = (long long)nx_ * ny_ * kzs_;
     nph_ = (long long)nx_ * ny_ * nz_;
-    load_tuning();
     GNSS_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     {
       int dev = 0;
@@ -390,11 +391,11 @@ class Engine : public EngineBase {
   }
   std::string tuning_string() const {
     char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d", cx_, cy_, zf_, xf_, cp_, xc_, zp_);
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d", cx_, cy_, zf_, xf_, cp_, xc_, zp_, kp_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_;
     const char* p = spec.c_str();
     const char* q;
     if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
@@ -404,16 +405,20 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "zf=")) || (q = strstr(p, "\"zf\": "))) zf = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "kp=")) || (q = strstr(p, "\"kp\": "))) kp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 3 ||
-        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1)
+        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
     zf_ = zf;
     xf_ = xf;
     cp_ = cp;
+    if (kp != kp_ && nsp_ > 0)
+      throw Error(GNSS_ERR_ARG, "kp (row padding) is fixed when the context is created: set it in the tuning table");
     xc_ = xc;
     zp_ = zp;
+    kp_ = kp;
   }
 
   // -------------------------------------------------------------------------------------------
@@ -557,7 +562,7 @@ class Engine : public EngineBase {
     {
       int best = zf_;
       double best_ms = 1e30;
-      for (int v = 0; v <= 4; ++v) {
+      for (int v = 0; v <= 5; ++v) {
         if (!zvariant_ok(v)) continue;
         zf_ = v;
         const double ms = time_it([&] { z_fused(); });
@@ -617,8 +622,8 @@ class Engine : public EngineBase {
   }
 
  private:
-  // fused z variants built: FP32 0-4, FP64 0 and 3
-  static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 4) : (v == 0 || v == 3); }
+  // fused z variants built: FP32 0-5, FP64 0 and 3
+  static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 5) : (v == 0 || v == 3); }
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
   bool xchunk_ok(int c) const {
     return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(nx_, c);
@@ -1065,6 +1070,7 @@ class Engine : public EngineBase {
       if (zf_ == 1) return zfused_launch_v<NZ, 1>(a0, scale);
       if (zf_ == 2) return zfused_launch_v<NZ, 2>(a0, scale);
       if (zf_ == 4) return zfused_launch_v<NZ, 4>(a0, scale);
+      if (zf_ == 5) return zfused_launch_v<NZ, 5>(a0, scale);
     }
     if (zf_ == 3) return zfused_launch_v<NZ, 3>(a0, scale);
     zfused_launch_v<NZ, 0>(a0, scale);
@@ -1321,7 +1327,7 @@ class Engine : public EngineBase {
   gnss_params p_;
   int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
-  int cp_ = 0, xc_ = 8, zp_ = 0, n_cu_ = 256;
+  int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, n_cu_ = 256;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

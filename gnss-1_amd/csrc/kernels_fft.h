@@ -381,31 +381,40 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
 // with the next rows' loads issued early (FP32), V = 1 the default plan (fewer radix stages, more
 // registers), V = 2 the fewest elements per thread without early loads (fewest registers),
 // V = 3 as 2 with a wider LDS row padding (fewer bank conflicts in the radix-4 exchanges: 5 entries
-// every 32 for 8-byte, 3 every 16 for 16-byte elements), V = 4 the default plan without early loads.
+// every 32 for 8-byte, 3 every 16 for 16-byte elements), V = 4 the default plan without early loads,
+// V = 5 as 3 with the register budget of four waves per SIMD.
 template <typename T, int M, int V> struct FusedPlan {
   using type = PlanLow<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
   using pad = LdsRow;
+  static constexpr int min_waves = 1;  // waves per SIMD the register budget must allow
 };
 template <typename T, int M> struct FusedPlan<T, M, 1> {
   using type = Plan<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
   using pad = LdsRow;
+  static constexpr int min_waves = 1;
 };
 template <typename T, int M> struct FusedPlan<T, M, 2> {
   using type = PlanLow<M>;
   static constexpr bool prefetch = false;
   using pad = LdsRow;
+  static constexpr int min_waves = 1;
 };
 template <typename T, int M> struct FusedPlan<T, M, 3> {
   using type = PlanLow<M>;
   static constexpr bool prefetch = false;
   using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<4, 3>>::type;
+  static constexpr int min_waves = 1;
 };
 template <typename T, int M> struct FusedPlan<T, M, 4> {
   using type = Plan<M>;
   static constexpr bool prefetch = false;
   using pad = LdsRow;
+  static constexpr int min_waves = 1;
+};
+template <typename T, int M> struct FusedPlan<T, M, 5> : FusedPlan<T, M, 3> {
+  static constexpr int min_waves = 4;
 };
 
 // Work rows: u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar, and
@@ -446,7 +455,8 @@ template <typename V> __device__ __forceinline__ V mul_pair(const V& a, const V&
 // Each row owns FUSED_B LDS regions of RS entries.
 constexpr int FUSED_B = 3;
 template <typename T, int NZ, int V>
-__global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E))
+__global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E),
+                                  (FusedPlan<T, NZ / 2, V>::min_waves))
 k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, T scale) {
   using FP = FusedPlan<T, NZ / 2, V>;
   using Z = ZRow<T, NZ, typename FP::type, typename FP::pad>;

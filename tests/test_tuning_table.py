@@ -13,11 +13,12 @@ def test_table_is_valid():
     for key, v in t.items():
         prec, nx, ny, nz, s = key.split()
         assert prec in ("f32", "f64") and int(s.rstrip("c")) >= (2 if s.endswith("c") else 0)
-        assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp"}
+        assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp", "kp"}
+        assert v.get("kp", 0) in range(5)
         allowed = {4, 8} if prec == "f64" else {8, 16}
         assert v["cx"] in allowed and v["cy"] in allowed and v.get("xc", v["cx"]) in allowed
         assert v["xf"] in (0, 1, 2, 3) and v["cp"] in (0, 1) and v.get("zp", 0) in (0, 1)
-        assert v["zf"] in ((0, 3) if prec == "f64" else (0, 1, 2, 3, 4))
+        assert v["zf"] in ((0, 3) if prec == "f64" else (0, 1, 2, 3, 4, 5))
     # one entry per line: the engine scans the file line by line for its key
     lines = [l for l in open(TABLE).read().splitlines() if l.startswith('"')]
     assert len(lines) == len(t)
