@@ -58,6 +58,7 @@ def tune_kp(cfg, table, tmp, steps=4):
     try:
         for kp in range(4):
             s = gnss.Solver(cfg)
+            s.init_wavenumbers()
             key = s.autotune(0)["key"]
             s.close()
             write_table(tmp, {key: {**table.get(key, {}), "kp": kp}})
