@@ -2,9 +2,9 @@
 writes the winning choices to tuning/gnss_tuning.json, which gnss_create reads.
 
 Tuned per (precision, grid, scalars, colloid): the column tile widths of the strided x and y
-passes, persistent or one-block-per-tile column and z passes, the fused z pass variant, the x-pass
-mode and its tile (`gnss_autotune` inside one context), and, for the 512^3 jets, the row padding
-kp (whole steps of fresh contexts, since kp fixes the array layout).
+passes, persistent or one-block-per-tile column and z passes, the fused z pass variant, and the
+x-pass mode and its tile (`gnss_autotune` inside one context).  The row padding kp keeps the value
+the table already holds (0 by default).
 
 usage: python gnss-1_amd/autotune.py [--reps 5] [--out gnss-1_amd/tuning/gnss_tuning.json] [--big]
 """
@@ -36,11 +36,6 @@ def cases(big: bool):
     return out
 
 
-def kp_cases():
-    """Workloads whose x-pass row stride is a large power of two times a small factor."""
-    return [gnss.config("jet512", precision=gnss.FP32), gnss.config("jet512")]
-
-
 def write_table(path, table):
     with open(path, "w") as f:
         f.write("{\n")
@@ -48,37 +43,6 @@ def write_table(path, table):
         for i, (k, v) in enumerate(items):
             f.write(f'"{k}": {json.dumps(v)}{"," if i + 1 < len(items) else ""}\n')
         f.write("}\n")
-
-
-def tune_kp(cfg, table, tmp, steps=4):
-    """Time `steps` RK3 steps of a fresh context per row padding kp = 0..3 (other choices from
-    `table`) and store the fastest kp in `table`."""
-    old = os.environ.get("GNSS_TUNING")
-    res, key = {}, None
-    try:
-        for kp in range(4):
-            s = gnss.Solver(cfg)
-            s.init_wavenumbers()
-            key = s.autotune(0)["key"]
-            s.close()
-            write_table(tmp, {key: {**table.get(key, {}), "kp": kp}})
-            os.environ["GNSS_TUNING"] = tmp
-            s = gnss.Solver(cfg).setup()
-            assert f"kp={kp}" in s.autotune(0)["choice"]
-            s.step(2)
-            t0 = time.perf_counter()
-            s.step(steps)
-            res[kp] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
-            s.close()
-    finally:
-        if old is None:
-            os.environ.pop("GNSS_TUNING", None)
-        else:
-            os.environ["GNSS_TUNING"] = old
-        if os.path.exists(tmp):
-            os.remove(tmp)
-    table.setdefault(key, {})["kp"] = min(res, key=res.get)
-    return key, res
 
 
 def main():
@@ -97,11 +61,6 @@ def main():
         table[rep["key"]] = {k: int(v) for k, v in choice.items()}
         reports.append(rep)
         print(f"[autotune] {rep['key']}: {rep['choice']}  ({time.time() - t0:.1f} s)", flush=True)
-    # row padding kp is fixed when a context is created: time whole steps of fresh contexts
-    for cfg in kp_cases():
-        key, res = tune_kp(cfg, table, args.out + ".kp.tmp")
-        reports.append({"key": key, "kp_ms_per_step": res})
-        print(f"[autotune] {key}: kp={table[key]['kp']}  {res}", flush=True)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     write_table(args.out, table)
     with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:
