@@ -391,11 +391,12 @@ class Engine : public EngineBase {
   }
   std::string tuning_string() const {
     char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d", cx_, cy_, zf_, xf_, cp_, xc_, zp_, kp_);
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d cb=%d", cx_, cy_, zf_, xf_, cp_, xc_, zp_,
+             kp_, cb_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_, cb = cb_;
     const char* p = spec.c_str();
     const char* q;
     if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
@@ -406,8 +407,9 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "xc=")) || (q = strstr(p, "\"xc\": "))) xc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "kp=")) || (q = strstr(p, "\"kp\": "))) kp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "cb=")) || (q = strstr(p, "\"cb\": "))) cb = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 3 ||
-        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4)
+        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
@@ -419,6 +421,7 @@ class Engine : public EngineBase {
     xc_ = xc;
     zp_ = zp;
     kp_ = kp;
+    cb_ = cb;
   }
 
   // -------------------------------------------------------------------------------------------
@@ -603,6 +606,17 @@ class Engine : public EngineBase {
         }
         xf_ = best;
         xc_ = best_xc;
+        // the fields of each pass in one launch, or one launch per field
+        int best_cb = cb_;
+        best_ms = 1e30;
+        for (int v = 0; v <= 1; ++v) {
+          cb_ = v;
+          const double ms = time_it([&] { substage(1); });
+          snprintf(buf, sizeof buf, ", \"cb=%d\": %.5f", v, ms);
+          rep += buf;
+          if (ms < best_ms) { best_ms = ms; best_cb = v; }
+        }
+        cb_ = best_cb;
         for (int f = 0; f < F_; ++f) {
           GNSS_HIP(hipMemcpyAsync(q_[f], snap[2 * f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
           GNSS_HIP(hipMemcpyAsync(dq_[f], snap[2 * f + 1], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
@@ -816,8 +830,9 @@ class Engine : public EngineBase {
     int in_keep, out_keep;
   };
 
+  // the pass over nf fields: persistent blocks over all their tiles (cp), else one launch per field
   template <int L, int SIGN, int CC>
-  void col_launch(C* data, const ColArgs& a, int G) {
+  void col_launch(C* const* fields, int nf, const ColArgs& a, int G) {
     constexpr int TPC = L / Plan<L>::E;
     const int threads = CC * TPC * G;
     const size_t lds = (size_t)G * L * CC * sizeof(C);
@@ -826,23 +841,26 @@ class Engine : public EngineBase {
       // persistent: as many blocks as stay resident (LDS and wave limits of a CU), all CUs
       const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds,
                                                                            2048 / threads));
-      const long long grid = std::min<long long>(blocks, per_cu * n_cu_);
-      hipLaunchKernelGGL((k_col_pipe<T, L, SIGN, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, data,
-                         tw_.at(L), a, (int)blocks);
+      const long long grid = std::min<long long>(blocks * nf, per_cu * n_cu_);
+      FieldList<T> fl{};
+      for (int f = 0; f < nf; ++f) fl.p[f] = fields[f];
+      hipLaunchKernelGGL((k_col_multi<T, L, SIGN, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, fl,
+                         tw_.at(L), a, (int)blocks, (int)(blocks * nf));
       return;
     }
-    hipLaunchKernelGGL((k_col<T, L, SIGN, CC>), dim3((unsigned)blocks), dim3(threads), lds, s_, data,
-                       tw_.at(L), a);
+    for (int f = 0; f < nf; ++f)
+      hipLaunchKernelGGL((k_col<T, L, SIGN, CC>), dim3((unsigned)blocks), dim3(threads), lds, s_, fields[f],
+                         tw_.at(L), a);
   }
 
   template <int L, int SIGN>
-  void col_dispatch_c(C* data, const ColArgs& a, int CC, int G) {
+  void col_dispatch_c(C* const* fields, int nf, const ColArgs& a, int CC, int G) {
     if constexpr (sizeof(T) == 8) {
-      if (CC == 8) col_launch<L, SIGN, 8>(data, a, G);
-      else col_launch<L, SIGN, 4>(data, a, G);
+      if (CC == 8) col_launch<L, SIGN, 8>(fields, nf, a, G);
+      else col_launch<L, SIGN, 4>(fields, nf, a, G);
     } else {
-      if (CC == 16) col_launch<L, SIGN, 16>(data, a, G);
-      else col_launch<L, SIGN, 8>(data, a, G);
+      if (CC == 16) col_launch<L, SIGN, 16>(fields, nf, a, G);
+      else col_launch<L, SIGN, 8>(fields, nf, a, G);
     }
   }
 
@@ -868,20 +886,20 @@ class Engine : public EngineBase {
   }
 
   template <int SIGN>
-  void col_dispatch_l(int L, C* data, const ColArgs& a, int CC, int G) {
+  void col_dispatch_l(int L, C* const* data, int nf, const ColArgs& a, int CC, int G) {
     switch (L) {
-      case 16: col_dispatch_c<16, SIGN>(data, a, CC, G); break;
-      case 24: col_dispatch_c<24, SIGN>(data, a, CC, G); break;
-      case 32: col_dispatch_c<32, SIGN>(data, a, CC, G); break;
-      case 48: col_dispatch_c<48, SIGN>(data, a, CC, G); break;
-      case 64: col_dispatch_c<64, SIGN>(data, a, CC, G); break;
-      case 96: col_dispatch_c<96, SIGN>(data, a, CC, G); break;
-      case 128: col_dispatch_c<128, SIGN>(data, a, CC, G); break;
-      case 192: col_dispatch_c<192, SIGN>(data, a, CC, G); break;
-      case 256: col_dispatch_c<256, SIGN>(data, a, CC, G); break;
-      case 384: col_dispatch_c<384, SIGN>(data, a, CC, G); break;
-      case 512: col_dispatch_c<512, SIGN>(data, a, CC, G); break;
-      case 1024: col_dispatch_c<1024, SIGN>(data, a, CC, G); break;
+      case 16: col_dispatch_c<16, SIGN>(data, nf, a, CC, G); break;
+      case 24: col_dispatch_c<24, SIGN>(data, nf, a, CC, G); break;
+      case 32: col_dispatch_c<32, SIGN>(data, nf, a, CC, G); break;
+      case 48: col_dispatch_c<48, SIGN>(data, nf, a, CC, G); break;
+      case 64: col_dispatch_c<64, SIGN>(data, nf, a, CC, G); break;
+      case 96: col_dispatch_c<96, SIGN>(data, nf, a, CC, G); break;
+      case 128: col_dispatch_c<128, SIGN>(data, nf, a, CC, G); break;
+      case 192: col_dispatch_c<192, SIGN>(data, nf, a, CC, G); break;
+      case 256: col_dispatch_c<256, SIGN>(data, nf, a, CC, G); break;
+      case 384: col_dispatch_c<384, SIGN>(data, nf, a, CC, G); break;
+      case 512: col_dispatch_c<512, SIGN>(data, nf, a, CC, G); break;
+      case 1024: col_dispatch_c<1024, SIGN>(data, nf, a, CC, G); break;
       default: throw Error(GNSS_ERR_UNSUPPORTED, "unsupported transform length " + std::to_string(L));
     }
   }
@@ -906,7 +924,10 @@ class Engine : public EngineBase {
     }
   }
 
-  void col_pass(C* data, const ColSpec& sp) {
+  void col_pass(C* data, const ColSpec& sp) { col_passes(&data, 1, sp); }
+
+  // The same strided pass over nf fields; with cb (and cp) one persistent launch covers them all.
+  void col_passes(C* const* fields, int nf, const ColSpec& sp) {
     const int L = sp.axis == 0 ? nx_ : ny_;
     const int CC = sp.axis == 0 ? cx_ : cy_;
     const int TPC = L / elems_per_thread(L);
@@ -938,16 +959,19 @@ class Engine : public EngineBase {
       a.G = 1;
       const char* mname = sp.axis == 0 ? (sp.sign < 0 ? "fft_x_fwd_mfma" : "fft_x_inv_mfma")
                                        : (sp.sign < 0 ? "fft_y_fwd_mfma" : "fft_y_inv_mfma");
-      timed(mname, bytes, [&] {
-        if (sp.sign < 0) mfma_dispatch<-1>(L, data, a, CC);
-        else mfma_dispatch<+1>(L, data, a, CC);
-      });
+      for (int f = 0; f < nf; ++f)
+        timed(mname, bytes, [&] {
+          if (sp.sign < 0) mfma_dispatch<-1>(L, fields[f], a, CC);
+          else mfma_dispatch<+1>(L, fields[f], a, CC);
+        });
       return;
     }
-    timed(name, bytes, [&] {
-      if (sp.sign < 0) col_dispatch_l<-1>(L, data, a, CC, G);
-      else col_dispatch_l<+1>(L, data, a, CC, G);
-    });
+    const int groups = cb_ ? 1 : nf, per = cb_ ? nf : 1;
+    for (int gi = 0; gi < groups; ++gi)
+      timed(name, bytes * per, [&] {
+        if (sp.sign < 0) col_dispatch_l<-1>(L, fields + gi * per, per, a, CC, G);
+        else col_dispatch_l<+1>(L, fields + gi * per, per, a, CC, G);
+      });
   }
 
   template <int NZ>
@@ -1115,7 +1139,7 @@ class Engine : public EngineBase {
   }
 
   template <int L, int CC>
-  void xop_launch(int w, ColArgs a) {
+  void xop_launch(const int* ws, int n, bool two, ColArgs a) {
     constexpr int TPC = L / Plan<L>::E;
     int G = std::max(1, 256 / (CC * TPC));
     while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
@@ -1124,29 +1148,53 @@ class Engine : public EngineBase {
     const size_t lds = (size_t)G * L * CC * sizeof(C);
     const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
     const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds, 2048 / threads));
-    const long long grid = std::min<long long>(blocks, per_cu * n_cu_);
-    const C* A;
-    const C* B;
-    const XOp op = xop_of(w, A, B);
-    if (op.kind == 2)
-      hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC, true>), dim3((unsigned)grid), dim3(threads), lds, s_, ws_[w],
-                         A, B, op, tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks);
+    XOpList<T> li{};
+    for (int i = 0; i < n; ++i) {
+      const C* A;
+      const C* B;
+      li.op[i] = xop_of(ws[i], A, B);
+      li.out[i] = ws_[ws[i]];
+      li.A[i] = A;
+      li.B[i] = B ? B : A;
+    }
+    const long long grid_all = std::min<long long>(blocks * n, per_cu * n_cu_);
+    if (two)
+      hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC, true>), dim3((unsigned)grid_all), dim3(threads), lds, s_, li,
+                         tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks, (int)(blocks * n));
     else
-      hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC, false>), dim3((unsigned)grid), dim3(threads), lds, s_, ws_[w],
-                         A, A, op, tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks);
+      hipLaunchKernelGGL((k_xinv_op_pipe<T, L, CC, false>), dim3((unsigned)grid_all), dim3(threads), lds, s_, li,
+                         tkx_, tky_, tkz_, tw_.at(L), a, (int)blocks, (int)(blocks * n));
   }
   template <int L>
-  void xop_c(int w, const ColArgs& a) {
+  void xop_c(const int* ws, int n, bool two, const ColArgs& a) {
     if constexpr (sizeof(T) == 8) {
-      if (xc_ == 8) xop_launch<L, 8>(w, a);
-      else xop_launch<L, 4>(w, a);
+      if (xc_ == 8) xop_launch<L, 8>(ws, n, two, a);
+      else xop_launch<L, 4>(ws, n, two, a);
     } else {
-      if (xc_ == 16) xop_launch<L, 16>(w, a);
-      else xop_launch<L, 8>(w, a);
+      if (xc_ == 16) xop_launch<L, 16>(ws, n, two, a);
+      else xop_launch<L, 8>(ws, n, two, a);
     }
   }
-  // x inverse pass of work field w formed from the state (mode 2)
-  void x_inv_op(int w) {
+  // x inverse passes of all work fields formed from the state (modes 2 and 3): one-input and
+  // two-input fields in one launch each with cb, else one launch per field
+  void x_inv_ops() {
+    std::vector<int> one, two;
+    for (int w = 0; w < W_; ++w) {
+      const C* A;
+      const C* B;
+      (xop_of(w, A, B).kind == 2 ? two : one).push_back(w);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      const std::vector<int>& v = pass ? two : one;
+      if (v.empty()) continue;
+      if (cb_) {
+        x_inv_op(v.data(), (int)v.size(), pass == 1);
+      } else {
+        for (int w : v) x_inv_op(&w, 1, pass == 1);
+      }
+    }
+  }
+  void x_inv_op(const int* ws, int n, bool two) {
     ColArgs a;
     a.estride = (long long)ny_ * kzs_;
     a.ostride = kzs_;
@@ -1157,24 +1205,22 @@ class Engine : public EngineBase {
     a.in_keep = g_.keep_x;
     a.out_keep = -1;
     a.G = 1;
-    const C* A;
-    const C* B;
-    const int nin = xop_of(w, A, B).kind == 2 ? 2 : 1;
-    const double bytes = (double)a.nouter * a.nchunks * xc_ * (nin * (2.0 * g_.keep_x + 1) + nx_) * sizeof(C);
+    const int nin = two ? 2 : 1;
+    const double bytes = n * (double)a.nouter * a.nchunks * xc_ * (nin * (2.0 * g_.keep_x + 1) + nx_) * sizeof(C);
     timed("fft_x_inv_op", bytes, [&] {
       switch (nx_) {
-        case 16: xop_c<16>(w, a); break;
-        case 24: xop_c<24>(w, a); break;
-        case 32: xop_c<32>(w, a); break;
-        case 48: xop_c<48>(w, a); break;
-        case 64: xop_c<64>(w, a); break;
-        case 96: xop_c<96>(w, a); break;
-        case 128: xop_c<128>(w, a); break;
-        case 192: xop_c<192>(w, a); break;
-        case 256: xop_c<256>(w, a); break;
-        case 384: xop_c<384>(w, a); break;
-        case 512: xop_c<512>(w, a); break;
-        case 1024: xop_c<1024>(w, a); break;
+        case 16: xop_c<16>(ws, n, two, a); break;
+        case 24: xop_c<24>(ws, n, two, a); break;
+        case 32: xop_c<32>(ws, n, two, a); break;
+        case 48: xop_c<48>(ws, n, two, a); break;
+        case 64: xop_c<64>(ws, n, two, a); break;
+        case 96: xop_c<96>(ws, n, two, a); break;
+        case 128: xop_c<128>(ws, n, two, a); break;
+        case 192: xop_c<192>(ws, n, two, a); break;
+        case 256: xop_c<256>(ws, n, two, a); break;
+        case 384: xop_c<384>(ws, n, two, a); break;
+        case 512: xop_c<512>(ws, n, two, a); break;
+        case 1024: xop_c<1024>(ws, n, two, a); break;
         default: throw Error(GNSS_ERR_UNSUPPORTED, "x pass length " + std::to_string(nx_));
       }
     });
@@ -1300,16 +1346,12 @@ class Engine : public EngineBase {
       });
       for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
     } else {
-      for (int w = 0; w < W_; ++w) {
-        if (xm >= 2) x_inv_op(w);
-        else if (xm == 0) col_pass(ws_[w], ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
-        col_pass(ws_[w], ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
-      }
+      if (xm >= 2) x_inv_ops();
+      else if (xm == 0) col_passes(ws_.data(), W_, ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+      col_passes(ws_.data(), W_, ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
       z_fused();
-      for (int f = 0; f < F_; ++f) {
-        col_pass(nl_[f], ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
-        if (!xfwd) col_pass(nl_[f], ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
-      }
+      col_passes(nl_.data(), F_, ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
+      if (!xfwd) col_passes(nl_.data(), F_, ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});
     }
     if (xfwd) {
       x_fused(false, -1, c);
@@ -1327,7 +1369,7 @@ class Engine : public EngineBase {
   gnss_params p_;
   int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
-  int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, n_cu_ = 256;
+  int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, cb_ = 0, n_cu_ = 256;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

@@ -21,6 +21,11 @@
 
 namespace gnss {
 
+// Work rows: u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar, and
+// with a colloid (the last scalar) two more: c itself and lap Z (Z = scalar 0).
+constexpr int MAXF = 3 + 4;           // velocity + up to 4 scalars
+constexpr int MAXW = 6 + 3 * 4 + 2;   // work rows of the largest field set
+
 // Outer-index map: identity, or the kept-mode set {0..keep} U {n-keep..n-1} of an axis.
 __device__ __forceinline__ int outer_index(int o, int keep, int n) {
   if (keep < 0) return o;
@@ -80,12 +85,18 @@ k_col(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a) {
   }
 }
 
-// Persistent, software-pipelined form of k_col: each block walks tiles blockIdx.x,
-// blockIdx.x + gridDim.x, ... and issues the HBM loads of its next tile before transforming the
-// current one (stage barriers wait only for LDS traffic, so the loads stay in flight).
+// Persistent, software-pipelined column pass over one or several fields: each block walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... and issues the HBM loads of its next tile before
+// transforming the current one (stage barriers wait only for LDS traffic, so the loads stay in
+// flight).  Tile i of the launch is tile i % tpf of field i / tpf: the fields of a pass share one
+// launch gap and one tail.
+template <typename T> struct FieldList {
+  cx<T>* p[MAXW];
+};
+
 template <typename T, int L, int SIGN, int C>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
-k_col_pipe(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a, int ntiles) {
+k_col_multi(FieldList<T> fl, const cx<T>* __restrict__ tw, ColArgs a, int tpf, int ntiles) {
   constexpr int E = Plan<L>::E;
   constexpr int TPC = L / E;
   extern __shared__ unsigned char smem_raw[];
@@ -95,11 +106,12 @@ k_col_pipe(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a, in
   const int t = (tid / C) % TPC;
   const int g = tid / (C * TPC);
   auto base_of = [&](int tile, bool& active) {
-    const int chunk = tile % a.nchunks;
-    const int o = (tile / a.nchunks) * a.G + g;
+    const int f = tile / tpf, lt = tile % tpf;
+    const int chunk = lt % a.nchunks;
+    const int o = (lt / a.nchunks) * a.G + g;
     active = o < a.nouter;
     const long long io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
-    return data + io * a.ostride + (long long)chunk * C + c;
+    return fl.p[f] + io * a.ostride + (long long)chunk * C + c;
   };
   auto load = [&](int tile, cx<T>* v) {
     bool active;
@@ -417,10 +429,6 @@ template <typename T, int M> struct FusedPlan<T, M, 5> : FusedPlan<T, M, 3> {
   static constexpr int min_waves = 4;
 };
 
-// Work rows: u, v, w, omega_x, omega_y, omega_z, (dphi/dx, dphi/dy, dphi/dz) per scalar, and
-// with a colloid (the last scalar) two more: c itself and lap Z (Z = scalar 0).
-constexpr int MAXF = 3 + 4;           // velocity + up to 4 scalars
-constexpr int MAXW = 6 + 3 * 4 + 2;   // work rows of the largest field set
 
 template <typename T> struct ZFusedArgs {
   const cx<T>* ws[MAXW];

@@ -518,16 +518,22 @@ k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, cons
 struct XOp {
   int kind, da, db;
 };
+template <typename T> struct XOpList {
+  cx<T>* out[MAXW];
+  const cx<T>* A[MAXW];
+  const cx<T>* B[MAXW];
+  XOp op[MAXW];
+};
 
-// Inverse x pass of one work field with its prologue folded into the loads: persistent and
-// software-pipelined like k_col_pipe (the next tile's inputs are loaded before the current tile
-// transforms).  TWO: the two-input operation (kind 2); otherwise only A is read.  Inputs and output share the spectral layout, so a column of A (and B) sits at the
+// Inverse x passes of work fields with their prologue folded into the loads: persistent and
+// software-pipelined like k_col_multi (the next tile's inputs are loaded before the current tile
+// transforms); tile i is tile i % tpf of list entry i / tpf.  TWO: the two-input operation
+// (kind 2); otherwise only A is read.  Inputs and output share the spectral layout, so a column of A (and B) sits at the
 // output column's offset; only kept kx rows are read, every row is written.
 template <typename T, int L, int C, bool TWO>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
-k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>* __restrict__ B, XOp op,
-               const T* __restrict__ kxv, const T* __restrict__ kyv, const T* __restrict__ kzv,
-               const cx<T>* __restrict__ tw, ColArgs a, int ntiles) {
+k_xinv_op_pipe(XOpList<T> li, const T* __restrict__ kxv, const T* __restrict__ kyv,
+               const T* __restrict__ kzv, const cx<T>* __restrict__ tw, ColArgs a, int tpf, int ntiles) {
   constexpr int E = Plan<L>::E;
   constexpr int TPC = L / E;
   extern __shared__ unsigned char smem_raw[];
@@ -537,8 +543,9 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
   const int t = (tid / C) % TPC;
   const int g = tid / (C * TPC);
   auto offset_of = [&](int tile, bool& active, int& iy, int& iz) {
-    const int chunk = tile % a.nchunks;
-    const int o = (tile / a.nchunks) * a.G + g;
+    const int lt = tile % tpf;
+    const int chunk = lt % a.nchunks;
+    const int o = (lt / a.nchunks) * a.G + g;
     active = o < a.nouter;
     iy = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
     iz = chunk * C + c;
@@ -546,7 +553,8 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
   };
   // A is loaded one tile ahead; B (two-input operation) when the tile is formed, which keeps the
   // register footprint of a 16-column tile without spills
-  auto load_col = [&](const cx<T>* __restrict__ src, int tile, cx<T>* va) {
+  auto load_col = [&](bool second, int tile, cx<T>* va) {
+    const cx<T>* __restrict__ src = second ? li.B[tile / tpf] : li.A[tile / tpf];
     bool active;
     int iy, iz;
     const long long off = offset_of(tile, active, iy, iz);
@@ -562,8 +570,9 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
     int iy, iz;
     (void)offset_of(tile, active, iy, iz);
     const T ky = kyv[iy], kz = kzv[iz];
+    const XOp op = li.op[tile / tpf];
     cx<T> vb[TWO ? E : 1];
-    if (TWO) load_col(B, tile, vb);
+    if (TWO) load_col(true, tile, vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const T kx = kxv[t + e * TPC];
@@ -580,18 +589,19 @@ k_xinv_op_pipe(cx<T>* __restrict__ out, const cx<T>* __restrict__ A, const cx<T>
   };
   int tile = blockIdx.x;
   cx<T> va[E], v[E];
-  load_col(A, tile, va);
+  load_col(false, tile, va);
   const LdsCol idx{g * L * C + c, C};
   while (true) {
     form(tile, va, v);
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
-    if (more) load_col(A, next, va);
+    if (more) load_col(false, next, va);
     fft<T, L, +1>(v, lds, t, idx, tw);
     bool active;
     int iy, iz;
     const long long off = offset_of(tile, active, iy, iz);
     if (active) {
+      cx<T>* __restrict__ out = li.out[tile / tpf];
 #pragma unroll
       for (int e = 0; e < E; ++e) out[off + (long long)(t + e * TPC) * a.estride] = v[e];
     }

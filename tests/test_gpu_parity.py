@@ -154,7 +154,7 @@ def test_per_field_x_inverse_matches(g, engine, prec):
     one FMA differently in the two kernels, so agreement is to rounding (FP64 1e-13, FP32 2e-6)."""
     cfg = g.with_colloid(g.Config(n=(64, 48, 96), re=400.0, sc=(0.7, 2.0), theta=0.08, noise_kmax=8.0,
                                   noise_amp=0.2, dt=1e-3, precision=prec), alpha=0.02)
-    for pair in (("xf=0", "xf=2"), ("xf=1", "xf=3")):
+    for pair in (("xf=0", "xf=2"), ("xf=1", "xf=3"), ("xf=2 cp=1 cb=0", "xf=2 cp=1 cb=1"), ("xf=0 cp=1 cb=0", "xf=0 cp=1 cb=1")):
         out = []
         for spec in pair:
             a = g.Solver(cfg, engine).setup()
