@@ -230,6 +230,10 @@ GNSS_API int gnss_timing_report(gnss_ctx* c, char* buf, int len) {
     memcpy(buf, s.c_str(), s.size() + 1);
   });
 }
+GNSS_API int gnss_bench_copy(gnss_ctx* c, int reps, double* ms, double* bytes) {
+  if (!ms || !bytes) return GNSS_ERR_ARG;
+  return guard(c, [&] { c->eng->bench_copy(reps, ms, bytes); });
+}
 GNSS_API int gnss_bench_fft(gnss_ctx* c, int reps, double* ms_fwd, double* ms_inv) {
   return guard(c, [&] { c->eng->bench_fft(reps, ms_fwd, ms_inv); });
 }

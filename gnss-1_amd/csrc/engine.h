@@ -73,6 +73,7 @@ class EngineBase {
   virtual void timing_enable(bool on) = 0;
   virtual std::string timing_report() = 0;
   virtual void bench_fft(int reps, double* ms_fwd, double* ms_inv) = 0;
+  virtual void bench_copy(int reps, double* ms, double* bytes) = 0;
   virtual std::string autotune(int reps) = 0;
   virtual void set_tuning(const std::string& spec) = 0;
   // Diagnostics: one physical plane of a field (or |omega|), and an iso-surface area.

@@ -344,6 +344,31 @@ class Engine : public EngineBase {
     return timer_.report();
   }
 
+  void bench_copy(int reps, double* ms, double* bytes) override {
+    if (reps <= 0) throw Error(GNSS_ERR_ARG, "reps must be > 0");
+    const long long n16 = (long long)(sizeof(C) * nsp_ / 16);
+    const long long grid = std::min<long long>(cdiv(n16, 256), (long long)n_cu_ * 32);
+    auto launch = [&] {
+      hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, s_, reinterpret_cast<const float4*>(ws_[0]),
+                         reinterpret_cast<float4*>(ws_[1]), n16);
+    };
+    hipEvent_t e0, e1;
+    GNSS_HIP(hipEventCreate(&e0));
+    GNSS_HIP(hipEventCreate(&e1));
+    launch();  // warm-up
+    GNSS_HIP(hipEventRecord(e0, s_));
+    for (int r = 0; r < reps; ++r) launch();
+    GNSS_HIP(hipEventRecord(e1, s_));
+    GNSS_HIP(hipEventSynchronize(e1));
+    GNSS_HIP(hipGetLastError());
+    float t = 0;
+    GNSS_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    *bytes = 2.0 * (double)n16 * 16.0;
+  }
+
   void bench_fft(int reps, double* ms_fwd, double* ms_inv) override {
     if (reps <= 0) throw Error(GNSS_ERR_ARG, "reps must be > 0");
     hipEvent_t e0, e1, e2;

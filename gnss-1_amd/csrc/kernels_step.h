@@ -702,6 +702,12 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   }
 }
 
+// Plain copy, 16 bytes per thread and iteration, grid-stride (bandwidth reference).
+static __global__ void k_copy16(const float4* __restrict__ in, float4* __restrict__ out, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Diagnostics
 // ---------------------------------------------------------------------------------------------

@@ -116,6 +116,7 @@ SIGNATURES = {
     "gnss_timing_enable": (_I, [_P, _I]),
     "gnss_timing_report": (_I, [_P, ctypes.c_char_p, _I]),
     "gnss_bench_fft": (_I, [_P, _I, _D, _D]),
+    "gnss_bench_copy": (_I, [_P, _I, _D, _D]),
     "gnss_autotune": (_I, [_P, _I, ctypes.c_char_p, _I]),
     "gnss_set_tuning": (_I, [_P, ctypes.c_char_p]),
     "gnss_ensemble_create": (_I, [_I, ctypes.POINTER(Params), ctypes.POINTER(_P)]),
@@ -349,6 +350,12 @@ class Solver:
 
     def set_tuning(self, spec: str):
         self._check(self.lib.gnss_set_tuning(self.ctx, spec.encode()))
+
+    def bench_copy(self, reps: int = 10):
+        """Mean ms of a plain device copy of one spectral field, and the bytes it reads + writes."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.gnss_bench_copy(self.ctx, int(reps), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def bench_fft(self, reps: int = 10):
         a, b = ctypes.c_double(), ctypes.c_double()

@@ -197,6 +197,11 @@ int gnss_timing_report(gnss_ctx* ctx, char* buf, int len);
 /* Times `reps` standalone forward+inverse 3D transform pairs of field 0 on the context's stream
  *   (engine only).  ms_fwd/ms_inv: average milliseconds per 3D R2C / C2R. */
 int gnss_bench_fft(gnss_ctx* ctx, int reps, double* ms_fwd, double* ms_inv);
+/* Times a plain device-to-device copy of one spectral work buffer into another (a vectorised
+ *   grid-stride kernel on the context's stream): the practical HBM bandwidth the passes are
+ *   measured against.  *ms = mean time per copy, *bytes = bytes read + written per copy.  No
+ *   reference counterpart (measurement); engine only. */
+int gnss_bench_copy(gnss_ctx* ctx, int reps, double* ms, double* bytes);
 
 /* ≈ the plan selection of plan1dFFT / plan2dFFT (fftfuncs.cu:133-231), which the reference leaves
  *   to cuFFT's planner.  The engine times its own compiled variants of the step's passes (column

@@ -1263,6 +1263,7 @@ EXPORT int gnss_surface_area(gnss_ctx* c, int field, double iso, double* area) {
 EXPORT int gnss_timing_enable(gnss_ctx* c, int e) { (void)e; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "no kernels on the CPU") : GNSS_ERR_ARG; }
 EXPORT int gnss_timing_report(gnss_ctx* c, char* b, int n) { (void)b; (void)n; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "no kernels on the CPU") : GNSS_ERR_ARG; }
 EXPORT int gnss_bench_fft(gnss_ctx* c, int r, double* a, double* b) { (void)r; (void)a; (void)b; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "engine only") : GNSS_ERR_ARG; }
+EXPORT int gnss_bench_copy(gnss_ctx* c, int r, double* a, double* b) { (void)r; (void)a; (void)b; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "engine only") : GNSS_ERR_ARG; }
 
 EXPORT int gnss_autotune(gnss_ctx* c, int r, char* b, int n) { (void)r; (void)b; (void)n; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "engine only") : GNSS_ERR_ARG; }
 EXPORT int gnss_set_tuning(gnss_ctx* c, const char* s) { (void)s; return c ? set_err(c, GNSS_ERR_UNSUPPORTED, "engine only") : GNSS_ERR_ARG; }

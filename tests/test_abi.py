@@ -2,6 +2,8 @@
 import ctypes
 import re
 
+import pytest
+
 from conftest import ENGINE_LIB, HEADER, ORACLE_LIB
 
 
@@ -41,3 +43,13 @@ def test_oracle_rejects_bad_parameters(g, oracle):
         with pytest.raises(g.GnssError) as e:
             g.Solver(g.Config(**bad), oracle)
         assert e.value.code == 1
+
+
+def test_oracle_reports_engine_only_calls_as_unsupported(g, oracle):
+    s = g.Solver(g.Config(n=(16, 16, 16), sc=(0.7,)), oracle)
+    s.init_wavenumbers()
+    for call in (lambda: s.bench_fft(1), lambda: s.bench_copy(1), lambda: s.timing(True),
+                 lambda: s.autotune(1)):
+        with pytest.raises(g.GnssError) as e:
+            call()
+        assert e.value.code == 6  # GNSS_ERR_UNSUPPORTED

@@ -85,3 +85,15 @@ def test_autotune_reports_and_applies(g, engine):
         s.set_tuning("cx=3")
     with pytest.raises(g.GnssError):
         s.set_tuning("xc=4")  # FP32 fused x kernels are built for tiles 8 and 16
+
+
+def test_bench_calls_report_rates(g, engine):
+    """The measurement entry points: a plain copy of one spectral field and a 3D FFT pair."""
+    s = g.Solver(g.Config(n=(128, 96, 64), sc=(0.7,), precision=g.FP32), engine).setup()
+    ms, nbytes = s.bench_copy(3)
+    kzs = -(-(64 // 2 + 1) // 16) * 16
+    assert nbytes == 2 * 128 * 96 * kzs * 8 and ms > 0
+    f, i = s.bench_fft(2)
+    assert f > 0 and i > 0
+    with pytest.raises(g.GnssError):
+        s.bench_copy(0)
