@@ -90,8 +90,18 @@ def dominant_kernel(rep):
                   "ms_per_launch": per_launch_ms}
 
 
-def cpu_baseline(cfg, threads):
-    """Bounded sample: one FP64 RK3 step of the same workload on the oracle (FFTW3 API)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(cfg, threads, steps=2):
+    """Bounded sample: `steps` FP64 RK3 steps of the same workload on the oracle (FFTW3 API)."""
     lib = gnss.load(ORACLE_LIB)
     lib.gnss_oracle_set_threads.argtypes = [gnss.ctypes.c_int]
     lib.gnss_oracle_set_backend.argtypes = [gnss.ctypes.c_int]
@@ -102,13 +112,13 @@ def cpu_baseline(cfg, threads):
     c64 = gnss.Config(**{**cfg.__dict__, "precision": gnss.FP64})
     o = gnss.Solver(c64, lib).setup()
     t0 = time.perf_counter()
-    o.step(1)
+    o.step(steps)
     dt = time.perf_counter() - t0
     name = lib.gnss_oracle_backend_name().decode()
     o.close()
-    return {"value": round(1.0 / dt, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"1 FP64 RK3 step of the same 512^3 workload on the oracle ({backend}: {name}); "
-                      f"{dt:.1f} s"}
+    return {"value": round(steps / dt, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} FP64 RK3 steps of the same 512^3 workload on the oracle ({backend}: {name}; "
+                      f"{threads} threads of {os.cpu_count()} on {cpu_model()}); {dt:.1f} s"}
 
 
 def main():
