@@ -347,7 +347,7 @@ class Engine : public EngineBase {
   void bench_copy(int reps, double* ms, double* bytes) override {
     if (reps <= 0) throw Error(GNSS_ERR_ARG, "reps must be > 0");
     const long long n16 = (long long)(sizeof(C) * nsp_ / 16);
-    const long long grid = std::min<long long>(cdiv(n16, 256), (long long)n_cu_ * 32);
+    const long long grid = std::min<long long>(cdiv(n16, 256 * 4), (long long)n_cu_ * 64);
     auto launch = [&] {
       hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, s_, reinterpret_cast<const float4*>(ws_[0]),
                          reinterpret_cast<float4*>(ws_[1]), n16);
@@ -615,32 +615,26 @@ class Engine : public EngineBase {
           GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
           GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), dq_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
         }
-        int best = xf_, best_xc = xc_;
+        // x-pass mode and tile, jointly with one launch per pass (cb = 1) or per field (cb = 0)
+        int best = xf_, best_xc = xc_, best_cb = cb_;
         double best_ms = 1e30;
-        for (int v = 0; v <= 3; ++v) {
-          for (int c : {4, 8, 16}) {
-            if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once
-            xf_ = v;
-            if (v) xc_ = c;
-            const double ms = time_it([&] { substage(1); });
-            if (v) snprintf(buf, sizeof buf, ", \"xf=%d xc=%d\": %.5f", v, c, ms);
-            else snprintf(buf, sizeof buf, ", \"xf=0\": %.5f", ms);
-            rep += buf;
-            if (ms < best_ms) { best_ms = ms; best = v; best_xc = xc_; }
+        for (int b = 0; b <= 1; ++b) {
+          for (int v = 0; v <= 3; ++v) {
+            for (int c : {4, 8, 16}) {
+              if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once per cb
+              cb_ = b;
+              xf_ = v;
+              if (v) xc_ = c;
+              const double ms = time_it([&] { substage(1); });
+              if (v) snprintf(buf, sizeof buf, ", \"cb=%d xf=%d xc=%d\": %.5f", b, v, c, ms);
+              else snprintf(buf, sizeof buf, ", \"cb=%d xf=0\": %.5f", b, ms);
+              rep += buf;
+              if (ms < best_ms) { best_ms = ms; best = v; best_xc = xc_; best_cb = b; }
+            }
           }
         }
         xf_ = best;
         xc_ = best_xc;
-        // the fields of each pass in one launch, or one launch per field
-        int best_cb = cb_;
-        best_ms = 1e30;
-        for (int v = 0; v <= 1; ++v) {
-          cb_ = v;
-          const double ms = time_it([&] { substage(1); });
-          snprintf(buf, sizeof buf, ", \"cb=%d\": %.5f", v, ms);
-          rep += buf;
-          if (ms < best_ms) { best_ms = ms; best_cb = v; }
-        }
         cb_ = best_cb;
         for (int f = 0; f < F_; ++f) {
           GNSS_HIP(hipMemcpyAsync(q_[f], snap[2 * f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));

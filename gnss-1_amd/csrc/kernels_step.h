@@ -702,10 +702,20 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   }
 }
 
-// Plain copy, 16 bytes per thread and iteration, grid-stride (bandwidth reference).
-static __global__ void k_copy16(const float4* __restrict__ in, float4* __restrict__ out, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    out[i] = in[i];
+// Plain copy (bandwidth reference): grid-stride, four 16-byte loads in flight per thread before
+// the stores.
+static __global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ in, float4* __restrict__ out,
+                                                       long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    float4 a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = in[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[i + u * stride] = a[u];
+  }
+  for (; i < n; i += stride) out[i] = in[i];
 }
 
 // ---------------------------------------------------------------------------------------------
