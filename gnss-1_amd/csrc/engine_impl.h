@@ -590,7 +590,7 @@ class Engine : public EngineBase {
     {
       int best = zf_;
       double best_ms = 1e30;
-      for (int v = 0; v <= 5; ++v) {
+      for (int v = 0; v <= 6; ++v) {
         if (!zvariant_ok(v)) continue;
         zf_ = v;
         const double ms = time_it([&] { z_fused(); });
@@ -656,7 +656,7 @@ class Engine : public EngineBase {
 
  private:
   // fused z variants built: FP32 0-5, FP64 0 and 3
-  static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 5) : (v == 0 || v == 3); }
+  static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 6) : (v == 0 || v == 3 || v == 6); }
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
   bool xchunk_ok(int c) const {
     return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(nx_, c);
@@ -1116,6 +1116,7 @@ class Engine : public EngineBase {
       if (zf_ == 5) return zfused_launch_v<NZ, 5>(a0, scale);
     }
     if (zf_ == 3) return zfused_launch_v<NZ, 3>(a0, scale);
+    if (zf_ == 6) return zfused_launch_v<NZ, 6>(a0, scale);
     zfused_launch_v<NZ, 0>(a0, scale);
   }
 

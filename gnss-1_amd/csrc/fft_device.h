@@ -164,6 +164,66 @@ template <int SIGN, typename V> __device__ __forceinline__ V twid(const V* __res
   return w;
 }
 
+// Number of radices of a plan and the radix of stage i.
+template <typename R> struct RadInfo;
+template <int... Rs> struct RadInfo<Radices<Rs...>> {
+  static constexpr int n = sizeof...(Rs);
+  static constexpr int r[sizeof...(Rs)] = {Rs...};
+};
+
+// The twiddles one thread of a length-L transform (plan P) multiplies by, over all radix stages,
+// read from the table once and then kept in registers.  A kernel that runs many transforms of one
+// length in the same thread positions (the fused z pass) so issues no twiddle load, and waits for
+// none, inside its radix stages.  Stage with span NS, block bt, input r >= 1 sits at
+// w[off(NS) + bt * (R - 1) + r - 1]; the values are exp(-2 pi i m / L) (conjugated at use).
+template <typename T, int L, typename P> struct TwRegs {
+  using RI = RadInfo<typename P::R>;
+  static constexpr int E = P::E;
+  static constexpr int off(int ns) {
+    int o = 0, s = 1;
+    for (int i = 0; i < RI::n; ++i) {
+      if (s == ns) return o;
+      if (s > 1) o += (E / RI::r[i]) * (RI::r[i] - 1);
+      s *= RI::r[i];
+    }
+    return o;
+  }
+  static constexpr int total = off(L);
+  cx<T> w[total > 0 ? total : 1];
+
+  __device__ __forceinline__ void load(const cx<T>* __restrict__ tw, int t) {
+    constexpr int TPC = L / E;
+    int s = 1, o = 0;
+#pragma unroll
+    for (int i = 0; i < RI::n; ++i) {
+      const int R = RI::r[i], NB = E / R;
+      if (s > 1) {
+#pragma unroll
+        for (int bt = 0; bt < NB; ++bt) {
+          const int k = (t + bt * TPC) % s;
+#pragma unroll
+          for (int r = 1; r < R; ++r) w[o + bt * (R - 1) + r - 1] = tw[r * k * (L / (s * R))];
+        }
+        o += NB * (R - 1);
+      }
+      s *= R;
+    }
+  }
+};
+
+// Twiddle of input r of block bt in the stage with span NS: from the table, or from registers.
+template <int SIGN, int L, int NS, int R, typename V>
+__device__ __forceinline__ V stage_tw(const V* __restrict__ tw, int k, int bt, int r) {
+  return twid<SIGN>(tw, r * k * (L / (NS * R)));
+}
+template <int SIGN, int L, int NS, int R, typename V, typename T, typename P>
+__device__ __forceinline__ V stage_tw(const TwRegs<T, L, P>& tw, int, int bt, int r) {
+  constexpr int o = TwRegs<T, L, P>::off(NS);
+  V w = tw.w[o + bt * (R - 1) + r - 1];
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
 // LDS index maps: column tile [pos][C] or row tile [row][pos] with padding.
 struct LdsCol {
   int base, C;
@@ -198,8 +258,8 @@ template <bool WAVE> __device__ __forceinline__ void lds_sync() {
 template <typename T, int L, int E, int SIGN, int NS, typename Idx, bool WAVE, int B, int BSTRIDE, int R,
           int... Rest>
 struct Stages {
-  __device__ __forceinline__ static void run(cx<T> (*v)[E], cx<T>* lds, int t, const Idx& idx,
-                                             const cx<T>* __restrict__ tw) {
+  template <typename TW>
+  __device__ __forceinline__ static void run(cx<T> (*v)[E], cx<T>* lds, int t, const Idx& idx, const TW& tw) {
     constexpr int TPC = L / E;
     constexpr int NB = E / R;
     constexpr bool LAST = sizeof...(Rest) == 0;
@@ -211,7 +271,7 @@ struct Stages {
       cx<T> w[R];
       if (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = twid<SIGN>(tw, r * k * (L / (NS * R)));
+        for (int r = 1; r < R; ++r) w[r] = stage_tw<SIGN, L, NS, R, cx<T>>(tw, k, bt, r);
       }
 #pragma unroll
       for (int b = 0; b < B; ++b) {
@@ -261,8 +321,8 @@ template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int 
 struct FftL;
 template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int B, int BSTRIDE, int... Rs>
 struct FftL<T, L, SIGN, Idx, P, WAVE, B, BSTRIDE, Radices<Rs...>> {
-  __device__ __forceinline__ static void run(cx<T> (*v)[P::E], cx<T>* lds, int t, const Idx& idx,
-                                             const cx<T>* __restrict__ tw) {
+  template <typename TW>
+  __device__ __forceinline__ static void run(cx<T> (*v)[P::E], cx<T>* lds, int t, const Idx& idx, const TW& tw) {
     Stages<T, L, P::E, SIGN, 1, Idx, WAVE, B, BSTRIDE, Rs...>::run(v, lds, t, idx, tw);
   }
 };
@@ -277,10 +337,10 @@ __device__ __forceinline__ void fft(cx<T>* v, cx<T>* lds, int t, const Idx& idx,
   FftL<T, L, SIGN, Idx, P, WAVE, 1, 0, typename P::R>::run(reinterpret_cast<cx<T>(*)[P::E]>(v), lds, t, idx, tw);
 }
 
-// B transforms at once; transform b exchanges through LDS region idx(pos) + b * BSTRIDE.
-template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int B, int BSTRIDE>
-__device__ __forceinline__ void fft_batch(cx<T> (*v)[P::E], cx<T>* lds, int t, const Idx& idx,
-                                          const cx<T>* __restrict__ tw) {
+// B transforms at once; transform b exchanges through LDS region idx(pos) + b * BSTRIDE.  The
+// twiddles come from the table (a pointer) or from a TwRegs of the same length and plan.
+template <typename T, int L, int SIGN, typename Idx, typename P, bool WAVE, int B, int BSTRIDE, typename TW>
+__device__ __forceinline__ void fft_batch(cx<T> (*v)[P::E], cx<T>* lds, int t, const Idx& idx, const TW& tw) {
   static_assert(!WAVE || (L / P::E) <= 64, "a wave-synchronised transform must fit one wavefront");
   FftL<T, L, SIGN, Idx, P, WAVE, B, BSTRIDE, typename P::R>::run(v, lds, t, idx, tw);
 }

@@ -162,10 +162,10 @@ struct ZRow {
   static constexpr int RS = Pad::stride(M);
   static_assert(64 % TPC == 0, "a z row must lie inside one wavefront");
 
-  // Registers of one spectral row as loaded from HBM (entries k >= lim read as 0).
+  // Registers of one spectral row as loaded from HBM (entries k >= lim read as 0).  A C2R ignores
+  // the imaginary parts of the entries k = 0 and k = M, so thread 0 keeps Re X[M] in x[0].y.
   struct Raw {
     cx<T> x[E];
-    cx<T> xm;  // entry k = M (thread t == 0)
   };
 
   __device__ __forceinline__ static void load(const cx<T>* __restrict__ src, bool active, int lim, int t,
@@ -175,8 +175,28 @@ struct ZRow {
       const int k = t + e * TPC;
       r.x[e] = (active && k < lim) ? src[k] : mk<T>(0, 0);
     }
-    r.xm = (active && t == 0 && M < lim) ? src[M] : mk<T>(0, 0);
+    if (t == 0) r.x[0].y = (active && M < lim) ? src[M].x : (T)0;
   }
+
+  // Twiddles of one thread for every row it transforms, read once and kept in registers: the
+  // radix stages' (TwRegs) and the packing factors twN[k] of its positions k = t + e TPC.  As
+  // (E/2) TPC = NZ/4, twN[k] of register e >= E/2 is -i times that of register e - E/2.
+  static_assert(E % 2 == 0, "register twiddles need an even number of elements per thread");
+  struct Tw {
+    TwRegs<T, M, P> m;
+    cx<T> n[E / 2];
+    __device__ __forceinline__ void load(const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, int t) {
+      m.load(twM, t);
+#pragma unroll
+      for (int e = 0; e < E / 2; ++e) n[e] = twN[t + e * TPC];
+    }
+  };
+  __device__ __forceinline__ static cx<T> ntw(const cx<T>* __restrict__ twN, int k, int) { return twN[k]; }
+  __device__ __forceinline__ static cx<T> ntw(const Tw& tw, int, int e) {
+    return e < E / 2 ? tw.n[e] : mk<T>(tw.n[e - E / 2].y, -tw.n[e - E / 2].x);
+  }
+  __device__ __forceinline__ static const cx<T>* mtw(const cx<T>* twM, const cx<T>*) { return twM; }
+  __device__ __forceinline__ static const TwRegs<T, M, P>& mtw(const Tw& tw, const Tw&) { return tw.m; }
 
   // Value of register `e` in lane TPC - t of this row (t = 0 reads its own lane): the partner
   // index M - k of k = t + e TPC lives there in register E - 1 - e.
@@ -191,21 +211,22 @@ struct ZRow {
   // C2R of B loaded rows to packed real rows, scaled by `scale` (Im of k = 0 and k = M ignored).
   // The pairing of k with M - k is a lane shuffle; only the radix stages use LDS, and the B rows
   // go through each stage together (LDS region of row b at idx + b * RS).
-  template <int B>
-  __device__ __forceinline__ static void c2r(const Raw* r, cx<T>* lds, const Pad& idx, int t,
-                                             const cx<T>* __restrict__ twM,
-                                             const cx<T>* __restrict__ twN, T scale, cx<T> (*v)[E]) {
+  // twM, twN: the tables (pointers), or both the same Tw.
+  template <int B, typename TM, typename TN>
+  __device__ __forceinline__ static void c2r(const Raw* r, cx<T>* lds, const Pad& idx, int t, const TM& twM,
+                                             const TN& twN, T scale, cx<T> (*v)[E]) {
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       cx<T> x[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = r[b].x[e];
+      const T xm = x[0].y;  // Re X[M] in thread 0
       if (t == 0) x[0].y = 0;
       cx<T> xc[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) xc[e] = mirror(x[E - 1 - e], t);
       if (t == 0) {
-        xc[0] = mk<T>(r[b].xm.x, (T)0);
+        xc[0] = mk<T>(xm, (T)0);
 #pragma unroll
         for (int e = 1; e < E; ++e) xc[e] = x[E - e];
       }
@@ -214,7 +235,7 @@ struct ZRow {
         const int k = t + e * TPC;
         const cx<T> xk = x[e];
         const cx<T> c = cconj(xc[e]);
-        const cx<T> p = cmul(cconj(twN[k]), csub(xk, c));
+        const cx<T> p = cmul(cconj(ntw(twN, k, e)), csub(xk, c));
         cx<T> z;
         z.x = xk.x + c.x - p.y;
         z.y = xk.y + c.y + p.x;
@@ -222,7 +243,7 @@ struct ZRow {
       }
     }
     lds_sync<true>();
-    fft_batch<T, M, +1, Pad, P, true, B, RS>(v, lds, t, idx, twM);
+    fft_batch<T, M, +1, Pad, P, true, B, RS>(v, lds, t, idx, mtw(twM, twN));
 #pragma unroll
     for (int b = 0; b < B; ++b)
 #pragma unroll
@@ -231,13 +252,12 @@ struct ZRow {
 
   // Transform B packed real rows to half spectra; entries k < lim of row b are stored to dst[b]
   // (the Nyquist value at k = M and zero padding up to kzs included).
-  template <int B>
+  template <int B, typename TM, typename TN>
   __device__ __forceinline__ static void r2c(cx<T> (*v)[E], cx<T>* const* dst, bool active, int lim,
-                                             int kzs, cx<T>* lds, const Pad& idx, int t,
-                                             const cx<T>* __restrict__ twM,
-                                             const cx<T>* __restrict__ twN) {
+                                             int kzs, cx<T>* lds, const Pad& idx, int t, const TM& twM,
+                                             const TN& twN) {
     lds_sync<true>();
-    fft_batch<T, M, -1, Pad, P, true, B, RS>(v, lds, t, idx, twM);
+    fft_batch<T, M, -1, Pad, P, true, B, RS>(v, lds, t, idx, mtw(twM, twN));
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       cx<T> zc[E];
@@ -255,7 +275,7 @@ struct ZRow {
           if (k < lim) {
             const cx<T> zk = v[b][e];
             const cx<T> c = cconj(zc[e]);
-            const cx<T> p = cmul(twN[k], csub(zk, c));
+            const cx<T> p = cmul(ntw(twN, k, e), csub(zk, c));
             cx<T> x;
             x.x = (T)0.5 * (zk.x + c.x + p.y);
             x.y = (T)0.5 * (zk.y + c.y - p.x);
@@ -400,33 +420,41 @@ template <typename T, int M, int V> struct FusedPlan {
   static constexpr bool prefetch = sizeof(T) == 4;
   using pad = LdsRow;
   static constexpr int min_waves = 1;  // waves per SIMD the register budget must allow
+  static constexpr bool regtw = false;  // twiddles read once per thread and kept in registers
 };
 template <typename T, int M> struct FusedPlan<T, M, 1> {
   using type = Plan<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
   using pad = LdsRow;
   static constexpr int min_waves = 1;
+  static constexpr bool regtw = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 2> {
   using type = PlanLow<M>;
   static constexpr bool prefetch = false;
   using pad = LdsRow;
   static constexpr int min_waves = 1;
+  static constexpr bool regtw = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 3> {
   using type = PlanLow<M>;
   static constexpr bool prefetch = false;
   using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<4, 3>>::type;
   static constexpr int min_waves = 1;
+  static constexpr bool regtw = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 4> {
   using type = Plan<M>;
   static constexpr bool prefetch = false;
   using pad = LdsRow;
   static constexpr int min_waves = 1;
+  static constexpr bool regtw = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 5> : FusedPlan<T, M, 3> {
   static constexpr int min_waves = 4;
+};
+template <typename T, int M> struct FusedPlan<T, M, 6> : FusedPlan<T, M, 3> {
+  static constexpr bool regtw = true;
 };
 
 
@@ -462,18 +490,13 @@ template <typename V> __device__ __forceinline__ V mul_pair(const V& a, const V&
 // and in FP32 the HBM loads of the next three rows are issued before the current ones transform.
 // Each row owns FUSED_B LDS regions of RS entries.
 constexpr int FUSED_B = 3;
-template <typename T, int NZ, int V>
-__global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E),
-                                  (FusedPlan<T, NZ / 2, V>::min_waves))
-k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, T scale) {
+template <typename T, int NZ, int V, typename TM, typename TN>
+__device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM, const TN& twN, T scale,
+                                           cx<T>* lds, int t, int g) {
   using FP = FusedPlan<T, NZ / 2, V>;
   using Z = ZRow<T, NZ, typename FP::type, typename FP::pad>;
   constexpr int E = Z::E;
   constexpr bool PREFETCH = FP::prefetch;
-  extern __shared__ unsigned char smem_raw[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int t = threadIdx.x % Z::TPC;
-  const int g = threadIdx.x / Z::TPC;
   const long long row = (long long)blockIdx.x * a.G + g;
   const bool active = row < a.nrows;
   const long long off = (active ? row : 0) * a.kzs;
@@ -557,6 +580,25 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
     for (int e = 0; e < E; ++e) fma_pair(n[0][e], mk<T>(a.alpha * om[1][e].x, a.alpha * om[1][e].y), om[0][e]);
     cx<T>* dst[1] = {a.nl[3 + sc] + off};
     Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+  }
+}
+
+template <typename T, int NZ, int V>
+__global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E),
+                                  (FusedPlan<T, NZ / 2, V>::min_waves))
+k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict__ twN, T scale) {
+  using FP = FusedPlan<T, NZ / 2, V>;
+  using Z = ZRow<T, NZ, typename FP::type, typename FP::pad>;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int t = threadIdx.x % Z::TPC;
+  const int g = threadIdx.x / Z::TPC;
+  if constexpr (FP::regtw) {
+    typename Z::Tw tw;
+    tw.load(twM, twN, t);
+    zfused_row<T, NZ, V>(a, tw, tw, scale, lds, t, g);
+  } else {
+    zfused_row<T, NZ, V>(a, twM, twN, scale, lds, t, g);
   }
 }
 
