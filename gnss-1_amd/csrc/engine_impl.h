@@ -655,7 +655,7 @@ class Engine : public EngineBase {
   }
 
  private:
-  // fused z variants built: FP32 0-5, FP64 0 and 3
+  // fused z variants built: FP32 0-6, FP64 0, 3 and 6
   static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 6) : (v == 0 || v == 3 || v == 6); }
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
   bool xchunk_ok(int c) const {

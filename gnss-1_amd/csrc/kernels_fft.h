@@ -414,7 +414,8 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
 // registers), V = 2 the fewest elements per thread without early loads (fewest registers),
 // V = 3 as 2 with a wider LDS row padding (fewer bank conflicts in the radix-4 exchanges: 5 entries
 // every 32 for 8-byte, 3 every 16 for 16-byte elements), V = 4 the default plan without early loads,
-// V = 5 as 3 with the register budget of four waves per SIMD.
+// V = 5 as 3 with the register budget of four waves per SIMD, V = 6 as 3 with each thread's
+// twiddles read once and held in registers (no twiddle load inside a radix stage).
 template <typename T, int M, int V> struct FusedPlan {
   using type = PlanLow<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
