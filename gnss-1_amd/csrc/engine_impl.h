@@ -590,7 +590,7 @@ class Engine : public EngineBase {
     {
       int best = zf_;
       double best_ms = 1e30;
-      for (int v = 0; v <= 6; ++v) {
+      for (int v = 0; v <= 8; ++v) {
         if (!zvariant_ok(v)) continue;
         zf_ = v;
         const double ms = time_it([&] { z_fused(); });
@@ -655,8 +655,10 @@ class Engine : public EngineBase {
   }
 
  private:
-  // fused z variants built: FP32 0-6, FP64 0, 3 and 6
-  static bool zvariant_ok(int v) { return sizeof(T) == 4 ? (v >= 0 && v <= 6) : (v == 0 || v == 3 || v == 6); }
+  // fused z variants built: FP32 0-7 (8 is 7 in FP32), FP64 0, 3, 6, 7 and 8
+  static bool zvariant_ok(int v) {
+    return sizeof(T) == 4 ? (v >= 0 && v <= 7) : (v == 0 || v == 3 || (v >= 6 && v <= 8));
+  }
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
   bool xchunk_ok(int c) const {
     return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(nx_, c);
@@ -1117,6 +1119,10 @@ class Engine : public EngineBase {
     }
     if (zf_ == 3) return zfused_launch_v<NZ, 3>(a0, scale);
     if (zf_ == 6) return zfused_launch_v<NZ, 6>(a0, scale);
+    if (zf_ == 7) return zfused_launch_v<NZ, 7>(a0, scale);
+    if constexpr (sizeof(T) == 8) {
+      if (zf_ == 8) return zfused_launch_v<NZ, 8>(a0, scale);
+    }
     zfused_launch_v<NZ, 0>(a0, scale);
   }
 

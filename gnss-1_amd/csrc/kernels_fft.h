@@ -173,7 +173,7 @@ struct ZRow {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + e * TPC;
-      r.x[e] = (active && k < lim) ? src[k] : mk<T>(0, 0);
+      r.x[e] = (active && k < lim) ? src[(unsigned)k] : mk<T>(0, 0);
     }
     if (t == 0) r.x[0].y = (active && M < lim) ? src[M].x : (T)0;
   }
@@ -208,27 +208,31 @@ struct ZRow {
     return r;
   }
 
-  // C2R of B loaded rows to packed real rows, scaled by `scale` (Im of k = 0 and k = M ignored).
-  // The pairing of k with M - k is a lane shuffle; only the radix stages use LDS, and the B rows
-  // go through each stage together (LDS region of row b at idx + b * RS).
-  // twM, twN: the tables (pointers), or both the same Tw.
-  template <int B, typename TM, typename TN>
+  // C2R of B loaded rows to packed real rows, scaled by `scale` unless SCALE is false (Im of k = 0
+  // and k = M ignored).  The pairing of k with M - k is a lane shuffle; thread 0, whose partners
+  // are its own registers, selects what it sends before the shuffle, so the shuffles of all B rows
+  // go out together.  Only the radix stages use LDS, and the B rows go through each stage together
+  // (LDS region of row b at idx + b * RS).  twM, twN: the tables (pointers), or both the same Tw.
+  template <int B, bool SCALE = true, typename TM, typename TN>
   __device__ __forceinline__ static void c2r(const Raw* r, cx<T>* lds, const Pad& idx, int t, const TM& twM,
                                              const TN& twN, T scale, cx<T> (*v)[E]) {
+    const bool t0 = t == 0;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       cx<T> x[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = r[b].x[e];
       const T xm = x[0].y;  // Re X[M] in thread 0
-      if (t == 0) x[0].y = 0;
+      x[0].y = t0 ? (T)0 : x[0].y;
       cx<T> xc[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) xc[e] = mirror(x[E - 1 - e], t);
-      if (t == 0) {
-        xc[0] = mk<T>(xm, (T)0);
-#pragma unroll
-        for (int e = 1; e < E; ++e) xc[e] = x[E - e];
+      for (int e = 0; e < E; ++e) {
+        // thread 0 pairs k = e TPC with M - k = (E - e) TPC, its own register E - e (M itself for e = 0)
+        const cx<T> own = e == 0 ? mk<T>(xm, (T)0) : x[E - e];
+        cx<T> snd = x[E - 1 - e];
+        snd.x = t0 ? own.x : snd.x;
+        snd.y = t0 ? own.y : snd.y;
+        xc[e] = mirror(snd, t);
       }
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -244,29 +248,34 @@ struct ZRow {
     }
     lds_sync<true>();
     fft_batch<T, M, +1, Pad, P, true, B, RS>(v, lds, t, idx, mtw(twM, twN));
+    if constexpr (SCALE) {
 #pragma unroll
-    for (int b = 0; b < B; ++b)
+      for (int b = 0; b < B; ++b)
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[b][e] = cscale(v[b][e], scale);
+        for (int e = 0; e < E; ++e) v[b][e] = cscale(v[b][e], scale);
+    }
   }
 
   // Transform B packed real rows to half spectra; entries k < lim of row b are stored to dst[b]
-  // (the Nyquist value at k = M and zero padding up to kzs included).
+  // (the Nyquist value at k = M and zero padding up to kzs included), multiplied by 2 h (h = 1/2:
+  // the plain transform; the fused z pass folds the inverse transforms' 1/N^2 into it).
   template <int B, typename TM, typename TN>
   __device__ __forceinline__ static void r2c(cx<T> (*v)[E], cx<T>* const* dst, bool active, int lim,
                                              int kzs, cx<T>* lds, const Pad& idx, int t, const TM& twM,
-                                             const TN& twN) {
+                                             const TN& twN, T h = (T)0.5) {
     lds_sync<true>();
     fft_batch<T, M, -1, Pad, P, true, B, RS>(v, lds, t, idx, mtw(twM, twN));
+    const bool t0 = t == 0;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       cx<T> zc[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) zc[e] = mirror(v[b][E - 1 - e], t);
-      if (t == 0) {
-        zc[0] = v[b][0];
-#pragma unroll
-        for (int e = 1; e < E; ++e) zc[e] = v[b][E - e];
+      for (int e = 0; e < E; ++e) {
+        const cx<T> own = v[b][(E - e) % E];  // thread 0: k = e TPC pairs with its register (E - e) mod E
+        cx<T> snd = v[b][E - 1 - e];
+        snd.x = t0 ? own.x : snd.x;
+        snd.y = t0 ? own.y : snd.y;
+        zc[e] = mirror(snd, t);
       }
       if (active) {
 #pragma unroll
@@ -277,12 +286,12 @@ struct ZRow {
             const cx<T> c = cconj(zc[e]);
             const cx<T> p = cmul(ntw(twN, k, e), csub(zk, c));
             cx<T> x;
-            x.x = (T)0.5 * (zk.x + c.x + p.y);
-            x.y = (T)0.5 * (zk.y + c.y - p.x);
-            dst[b][k] = x;
+            x.x = h * (zk.x + c.x + p.y);
+            x.y = h * (zk.y + c.y - p.x);
+            dst[b][(unsigned)k] = x;
           }
         }
-        if (t == 0 && M < lim) dst[b][M] = mk<T>(v[b][0].x - v[b][0].y, (T)0);
+        if (t0 && M < lim) dst[b][M] = mk<T>(((T)2 * h) * (v[b][0].x - v[b][0].y), (T)0);
         for (int k = M + 1 + t; k < kzs && k < lim; k += TPC) dst[b][k] = mk<T>(0, 0);
       }
     }
@@ -415,13 +424,16 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
 // V = 3 as 2 with a wider LDS row padding (fewer bank conflicts in the radix-4 exchanges: 5 entries
 // every 32 for 8-byte, 3 every 16 for 16-byte elements), V = 4 the default plan without early loads,
 // V = 5 as 3 with the register budget of four waves per SIMD, V = 6 as 3 with each thread's
-// twiddles read once and held in registers (no twiddle load inside a radix stage).
+// twiddles read once and held in registers (no twiddle load inside a radix stage), V = 7 as 6 with
+// the inverse transforms' 1/N moved to the forward ones, V = 8 as 7 with the 16-byte rows padded
+// by 2 entries every 8 (a 32-bank model of the FP64 exchanges puts fewer conflicts there).
 template <typename T, int M, int V> struct FusedPlan {
   using type = PlanLow<M>;
   static constexpr bool prefetch = sizeof(T) == 4;
   using pad = LdsRow;
   static constexpr int min_waves = 1;  // waves per SIMD the register budget must allow
   static constexpr bool regtw = false;  // twiddles read once per thread and kept in registers
+  static constexpr bool defer = false;  // 1/N of the inverse transforms applied as 1/N^2 on the products
 };
 template <typename T, int M> struct FusedPlan<T, M, 1> {
   using type = Plan<M>;
@@ -429,6 +441,7 @@ template <typename T, int M> struct FusedPlan<T, M, 1> {
   using pad = LdsRow;
   static constexpr int min_waves = 1;
   static constexpr bool regtw = false;
+  static constexpr bool defer = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 2> {
   using type = PlanLow<M>;
@@ -436,6 +449,7 @@ template <typename T, int M> struct FusedPlan<T, M, 2> {
   using pad = LdsRow;
   static constexpr int min_waves = 1;
   static constexpr bool regtw = false;
+  static constexpr bool defer = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 3> {
   using type = PlanLow<M>;
@@ -443,6 +457,7 @@ template <typename T, int M> struct FusedPlan<T, M, 3> {
   using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<4, 3>>::type;
   static constexpr int min_waves = 1;
   static constexpr bool regtw = false;
+  static constexpr bool defer = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 4> {
   using type = Plan<M>;
@@ -450,12 +465,19 @@ template <typename T, int M> struct FusedPlan<T, M, 4> {
   using pad = LdsRow;
   static constexpr int min_waves = 1;
   static constexpr bool regtw = false;
+  static constexpr bool defer = false;
 };
 template <typename T, int M> struct FusedPlan<T, M, 5> : FusedPlan<T, M, 3> {
   static constexpr int min_waves = 4;
 };
 template <typename T, int M> struct FusedPlan<T, M, 6> : FusedPlan<T, M, 3> {
   static constexpr bool regtw = true;
+};
+template <typename T, int M> struct FusedPlan<T, M, 7> : FusedPlan<T, M, 6> {
+  static constexpr bool defer = true;
+};
+template <typename T, int M> struct FusedPlan<T, M, 8> : FusedPlan<T, M, 7> {
+  using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<3, 2>>::type;
 };
 
 
@@ -498,6 +520,11 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   using Z = ZRow<T, NZ, typename FP::type, typename FP::pad>;
   constexpr int E = Z::E;
   constexpr bool PREFETCH = FP::prefetch;
+  // Variants with `defer`: the inverse transforms skip their 1/N; every product below is quadratic
+  // in the transformed fields, so the forward transforms take 1/N^2 instead (h = 1/(2 N^2) in place
+  // of 1/2, no extra instruction).
+  constexpr bool DEFER = FP::defer;
+  const T h = DEFER ? (T)0.5 * scale * scale : (T)0.5;
   const long long row = (long long)blockIdx.x * a.G + g;
   const bool active = row < a.nrows;
   const long long off = (active ? row : 0) * a.kzs;
@@ -510,7 +537,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
 #pragma unroll
     for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
   }
-  Z::template c2r<3>(r0, lds, idx, t, twM, twN, scale, vel);  // u, v, w
+  Z::template c2r<3, !DEFER>(r0, lds, idx, t, twM, twN, scale, vel);  // u, v, w
   if (!PREFETCH) {
 #pragma unroll
     for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
@@ -523,7 +550,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
 #pragma unroll
     for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s_first + b] + off, active, a.lim, t, r0[b]);
   }
-  Z::template c2r<3>(r1, lds, idx, t, twM, twN, scale, om);  // omega
+  Z::template c2r<3, !DEFER>(r1, lds, idx, t, twM, twN, scale, om);  // omega
   // u x omega, in place of omega
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -535,7 +562,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   }
   {
     cx<T>* dst[3] = {a.nl[0] + off, a.nl[1] + off, a.nl[2] + off};
-    Z::template r2c<3>(om, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+    Z::template r2c<3>(om, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN, h);
   }
   for (int j = 0; j < P; ++j) {
     const int s = a.colloid ? (j + 1 < P ? j + 1 : 0) : j;
@@ -543,7 +570,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
 #pragma unroll
       for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s + b] + off, active, a.lim, t, r0[b]);
     }
-    Z::template c2r<3>(r0, lds, idx, t, twM, twN, scale, om);  // grad phi
+    Z::template c2r<3, !DEFER>(r0, lds, idx, t, twM, twN, scale, om);  // grad phi
     cx<T> n[1][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -558,7 +585,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
         for (int e = 0; e < E; ++e) fma_pair(vel[b][e], mk<T>(a.alpha, a.alpha), om[b][e]);
     }
     cx<T>* dst[1] = {a.nl[3 + s] + off};
-    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN, h);
   }
   if (a.colloid) {
     // colloidAdvectionKernel_mgpu (solver.cu:374-391): u.grad c + alpha (lap Z c + grad Z . grad c)
@@ -566,7 +593,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
     const int sc = a.S - 1;
 #pragma unroll
     for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * sc + b] + off, active, a.lim, t, r0[b]);
-    Z::template c2r<3>(r0, lds, idx, t, twM, twN, scale, om);  // grad c
+    Z::template c2r<3, !DEFER>(r0, lds, idx, t, twM, twN, scale, om);  // grad c
     cx<T> n[1][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -576,11 +603,11 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
     }
 #pragma unroll
     for (int b = 0; b < 2; ++b) Z::load(a.ws[6 + 3 * a.S + b] + off, active, a.lim, t, r0[b]);
-    Z::template c2r<2>(r0, lds, idx, t, twM, twN, scale, om);  // c, lap Z
+    Z::template c2r<2, !DEFER>(r0, lds, idx, t, twM, twN, scale, om);  // c, lap Z
 #pragma unroll
     for (int e = 0; e < E; ++e) fma_pair(n[0][e], mk<T>(a.alpha * om[1][e].x, a.alpha * om[1][e].y), om[0][e]);
     cx<T>* dst[1] = {a.nl[3 + sc] + off};
-    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN);
+    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN, h);
   }
 }
 
@@ -593,7 +620,7 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
   extern __shared__ unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int t = threadIdx.x % Z::TPC;
-  const int g = threadIdx.x / Z::TPC;
+  const int g = Z::TPC == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x / 64) : threadIdx.x / Z::TPC;
   if constexpr (FP::regtw) {
     typename Z::Tw tw;
     tw.load(twM, twN, t);

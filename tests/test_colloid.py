@@ -169,7 +169,8 @@ def test_gpu_colloid_fp32_vs_fp64(g, engine):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,spec", [(0, "xf=0"), (0, "xf=0 cp=1"), (0, "xf=1 xc=4"), (0, "zf=3"), (1, "xf=1 zf=1"), (1, "zf=3"), (1, "zf=4"), (0, "xf=2"), (1, "xf=2 zf=3 xc=8"), (0, "xf=3"), (1, "xf=3 xc=16"), (0, "xf=2 cp=1 cb=1"), (1, "xf=2 cp=1 cb=1"), (1, "xf=0 cp=1 cb=1"),
-                                       (1, "xf=1 zf=2 xc=8"), (1, "xf=0 cp=1"), (0, "zf=6 xf=3"), (1, "zf=6 xf=2 cp=1 cb=1")])
+                                       (1, "xf=1 zf=2 xc=8"), (1, "xf=0 cp=1"), (0, "zf=6 xf=3"), (1, "zf=6 xf=2 cp=1 cb=1"),
+                                       (0, "zf=8 xf=2"), (1, "zf=7 xf=2 cp=1 cb=1")])
 def test_gpu_colloid_tuning_variants(g, engine, prec, spec):
     """Every pass variant computes the same step (FP64: to rounding; FP32: to FP32 rounding)."""
     cfg = colloid_jet(g, precision=prec)

@@ -50,11 +50,12 @@ def test_tuning_variants_agree(g, engine, prec):
                    precision=p)
     specs = ["cx=4 cy=4 zf=0 xf=0 cp=0", "cx=8 cy=8 zf=0 xf=1 cp=1 xc=8", "cx=4 cy=8 zf=0 xf=1 cp=0 xc=4",
              "cx=8 cy=4 zf=3 xf=0 cp=1 zp=1", "cx=4 cy=8 xf=2 xc=4 cb=1 cp=1", "cx=8 cy=8 xf=3 xc=8 cb=1", "cx=8 cy=8 xf=0 cp=1 cb=1",
-             "cx=8 cy=8 zf=6 xf=3 cb=1"] if prec == "fp64" else \
+             "cx=8 cy=8 zf=6 xf=3 cb=1", "cx=8 cy=8 zf=7 xf=2 cb=1", "cx=8 cy=8 zf=8 xf=0 cp=1 cb=1"] if prec == "fp64" else \
         ["cx=8 cy=8 zf=0 xf=0 cp=0", "cx=16 cy=16 zf=0 xf=1 cp=1 xc=16", "cx=16 cy=8 zf=1 xf=1 cp=0 xc=8",
          "cx=8 cy=16 zf=1 xf=0 cp=1 zp=1", "cx=16 cy=16 zf=2 xf=0 cp=1", "cx=16 cy=16 zf=3 xf=1 cp=1",
          "cx=16 cy=16 zf=4 xf=0 cp=0", "cx=16 cy=8 zf=3 xf=2 xc=8", "cx=8 cy=16 xf=2 xc=16", "cx=16 cy=16 zf=3 xf=3 xc=8", "cx=16 cy=16 zf=5 xf=2 xc=16", "cx=16 cy=16 zf=3 xf=2 xc=16 cp=1 cb=1",
-         "cx=8 cy=16 xf=0 cp=1 cb=1", "cx=16 cy=16 zf=6 xf=2 xc=16 cp=1 zp=1 cb=1"]
+         "cx=8 cy=16 xf=0 cp=1 cb=1", "cx=16 cy=16 zf=6 xf=2 xc=16 cp=1 zp=1 cb=1",
+         "cx=16 cy=16 zf=7 xf=3 xc=16 cb=1"]
     ref = None
     for spec in specs:
         s = g.Solver(cfg, engine).setup()

@@ -151,20 +151,23 @@ def test_fused_matches_unfused(g, engine, prec):
 def test_fused_register_twiddles(g, engine, prec, nz):
     """Fused z variant 6 reads each thread's twiddles once and keeps them in registers (the packing
     factors of registers e >= E/2 taken as -i times those of e - E/2); variant 3 reads them from
-    the tables inside every radix stage.  Both must give the same step, for every z length the
-    engine supports, with a colloid so that every branch of the kernel runs."""
+    the tables inside every radix stage; variant 7 also moves the inverse transforms' 1/N onto the
+    forward ones as 1/N^2, and the FP64 variant 8 pads its LDS rows differently.  All must give the
+    same step, for every z length the engine supports, with a colloid so that every branch of the
+    kernel runs."""
     cfg = g.with_colloid(g.Config(n=(32, 16, nz), re=400.0, sc=(0.7, 2.0), theta=0.08, noise_kmax=6.0,
                                   noise_amp=0.2, dt=1e-3, precision=prec), alpha=0.02)
     print(f"\n[register twiddles nz={nz} prec={prec}]", flush=True)
     out = []
-    for zf in (3, 6):
+    for zf in ((3, 6, 7, 8) if prec == 0 else (3, 6, 7)):
         a = g.Solver(cfg, engine).setup()
         a.set_tuning(f"zf={zf}")
         a.step(2)
         out.append([a.get_spectral(f) for f in range(5)])
         a.close()
-    for f in range(5):
-        assert rel(out[1][f], out[0][f]) <= (1e-13 if prec == 0 else 2e-6), (nz, f)
+    for o in out[1:]:
+        for f in range(5):
+            assert rel(o[f], out[0][f]) <= (1e-13 if prec == 0 else 2e-6), (nz, f)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
