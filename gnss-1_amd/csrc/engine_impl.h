@@ -433,7 +433,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "kp=")) || (q = strstr(p, "\"kp\": "))) kp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cb=")) || (q = strstr(p, "\"cb\": "))) cb = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
-    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > 3 ||
+    if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > max_xmode() ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
@@ -619,9 +619,10 @@ class Engine : public EngineBase {
         int best = xf_, best_xc = xc_, best_cb = cb_;
         double best_ms = 1e30;
         for (int b = 0; b <= 1; ++b) {
-          for (int v = 0; v <= 3; ++v) {
+          for (int v = 0; v <= max_xmode(); ++v) {
             for (int c : {4, 8, 16}) {
               if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once per cb
+              if (v >= 4 && sizeof(T) == 4 && c != 8) continue;  // FP32 late passes use 8-column tiles
               cb_ = b;
               xf_ = v;
               if (v) xc_ = c;
@@ -1147,6 +1148,10 @@ class Engine : public EngineBase {
   // work field on the fly (forward side as in mode 0); 3 the inverse side of mode 2 with the
   // forward side of mode 1.  The MFMA radix-16 path uses mode 0.
   int x_mode() const { return (p_.flags & GNSS_FLAG_MFMA_R16) ? 0 : xf_; }
+  // modes 4 and 5: late derivatives (the x pass reads each state field once, the y and z passes
+  // apply their own derivatives; see late_inverse), with mode 2's and mode 3's forward side.
+  // Not built for the colloid, whose work fields need second derivatives.
+  int max_xmode() const { return colloid_ ? 3 : 5; }
 
   // Work field w of a substage as an operation on one or two state fields.
   XOp xop_of(int w, const C*& A, const C*& B) const {
@@ -1201,6 +1206,126 @@ class Engine : public EngineBase {
       else xop_launch<L, 8>(ws, n, two, a);
     }
   }
+  // Late-derivative inverse passes (x modes 4, 5).  Work buffers: ws0 u, ws1 v, ws2 w, ws3 w_y,
+  // ws4 w_x, ws5 v_x (then omega_z), per scalar s ws[6+3s] phi_x, ws[7+3s] phi_y, ws[8+3s] phi.
+  // The x pass reads u, v, w, phi once each (4 inputs, 4 + 2 + S outputs instead of 12 + 3S
+  // inputs and 6 + 3S outputs); the y pass reads 5 + 2S fields and writes 6 + 3S, adding w_y,
+  // phi_y and omega_z = v_x - i k_y u; the fused z pass forms omega_x = w_y - i k_z v,
+  // omega_y = i k_z u - w_x and phi_z = i k_z phi as it loads.
+  void late_inverse() {
+    const PairOp copy{1, 1, {1, 0}, {0, 0}, {0, 0}, {0, 0}};        // out0 = A
+    const PairOp copy_d{1, 2, {1, 0}, {0, 1}, {0, 0}, {0, 0}};      // out0 = A, out1 = i k A
+    const PairOp curl_z{2, 2, {1, 0}, {0, -1}, {0, 1}, {0, 0}};     // out0 = A, out1 = B - i k A
+    PairList<T> lx{}, ly{};
+    int nxe = 0, nye = 0;
+    auto put = [](PairList<T>& l, int& n, const C* A, const C* B, C* o0, C* o1, const PairOp& op) {
+      l.A[n] = A;
+      l.B[n] = B ? B : A;
+      l.out[0][n] = o0;
+      l.out[1][n] = o1 ? o1 : o0;
+      l.op[n] = op;
+      ++n;
+    };
+    put(lx, nxe, q_[0], nullptr, ws_[0], nullptr, copy);
+    put(lx, nxe, q_[1], nullptr, ws_[1], ws_[5], copy_d);
+    put(lx, nxe, q_[2], nullptr, ws_[2], ws_[4], copy_d);
+    for (int q = 0; q < S_; ++q) put(lx, nxe, q_[3 + q], nullptr, ws_[8 + 3 * q], ws_[6 + 3 * q], copy_d);
+    put(ly, nye, ws_[0], ws_[5], ws_[0], ws_[5], curl_z);
+    put(ly, nye, ws_[1], nullptr, ws_[1], nullptr, copy);
+    put(ly, nye, ws_[2], nullptr, ws_[2], ws_[3], copy_d);
+    put(ly, nye, ws_[4], nullptr, ws_[4], nullptr, copy);
+    for (int q = 0; q < S_; ++q) {
+      put(ly, nye, ws_[8 + 3 * q], nullptr, ws_[8 + 3 * q], ws_[7 + 3 * q], copy_d);
+      put(ly, nye, ws_[6 + 3 * q], nullptr, ws_[6 + 3 * q], nullptr, copy);
+    }
+    const int CC = pair_chunk();
+    ColArgs ax;
+    ax.estride = (long long)ny_ * kzs_;
+    ax.ostride = kzs_;
+    ax.outer_keep = g_.keep_y;
+    ax.outer_n = ny_;
+    ax.nouter = 2 * g_.keep_y + 1;
+    ax.nchunks = kz_lim_ / CC;
+    ax.in_keep = g_.keep_x;
+    ax.out_keep = -1;
+    ax.G = 1;
+    ColArgs ay;
+    ay.estride = kzs_;
+    ay.ostride = (long long)ny_ * kzs_;
+    ay.outer_keep = -1;
+    ay.outer_n = nx_;
+    ay.nouter = nx_;
+    ay.nchunks = kz_lim_ / CC;
+    ay.in_keep = g_.keep_y;
+    ay.out_keep = -1;
+    ay.G = 1;
+    pair_pass(lx, nxe, ax, nx_, tkx_, "fft_x_inv_late");
+    pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late");
+  }
+  int pair_chunk() const { return sizeof(T) == 4 ? 8 : xc_; }
+  void pair_pass(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name) {
+    const int CC = pair_chunk();
+    auto rows = [&](int keep) { return keep < 0 ? L : std::min(L, 2 * keep + 1); };
+    const double col = (double)a.nouter * a.nchunks * CC * sizeof(C);
+    auto bytes_of = [&](int i0, int i1) {
+      double b = 0;
+      for (int i = i0; i < i1; ++i) b += col * (li.op[i].nin * rows(a.in_keep) + li.op[i].nout * rows(a.out_keep));
+      return b;
+    };
+    const int groups = cb_ ? 1 : n, per = cb_ ? n : 1;
+    for (int gi = 0; gi < groups; ++gi) {
+      PairList<T> sub{};
+      for (int i = 0; i < per; ++i) {
+        const int j = gi * per + i;
+        sub.A[i] = li.A[j];
+        sub.B[i] = li.B[j];
+        sub.out[0][i] = li.out[0][j];
+        sub.out[1][i] = li.out[1][j];
+        sub.op[i] = li.op[j];
+      }
+      timed(name, bytes_of(gi * per, gi * per + per), [&] {
+        switch (L) {
+          case 16: pair_c<16>(sub, per, a, kv, CC); break;
+          case 24: pair_c<24>(sub, per, a, kv, CC); break;
+          case 32: pair_c<32>(sub, per, a, kv, CC); break;
+          case 48: pair_c<48>(sub, per, a, kv, CC); break;
+          case 64: pair_c<64>(sub, per, a, kv, CC); break;
+          case 96: pair_c<96>(sub, per, a, kv, CC); break;
+          case 128: pair_c<128>(sub, per, a, kv, CC); break;
+          case 192: pair_c<192>(sub, per, a, kv, CC); break;
+          case 256: pair_c<256>(sub, per, a, kv, CC); break;
+          case 384: pair_c<384>(sub, per, a, kv, CC); break;
+          case 512: pair_c<512>(sub, per, a, kv, CC); break;
+          case 1024: pair_c<1024>(sub, per, a, kv, CC); break;
+          default: throw Error(GNSS_ERR_UNSUPPORTED, "late pass length " + std::to_string(L));
+        }
+      });
+    }
+  }
+  template <int L>
+  void pair_c(const PairList<T>& li, int n, ColArgs a, const T* kv, int CC) {
+    if constexpr (sizeof(T) == 8) {
+      if (CC == 8) pair_launch<L, 8>(li, n, a, kv);
+      else pair_launch<L, 4>(li, n, a, kv);
+    } else {
+      pair_launch<L, 8>(li, n, a, kv);
+    }
+  }
+  template <int L, int CC>
+  void pair_launch(const PairList<T>& li, int n, ColArgs a, const T* kv) {
+    constexpr int TPC = L / Plan<L>::E;
+    int G = std::max(1, 256 / (CC * TPC));
+    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
+    a.G = G;
+    const int threads = CC * TPC * G;
+    const size_t lds = (size_t)G * L * CC * sizeof(C);
+    const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
+    const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds, 2048 / threads));
+    const long long grid = std::min<long long>(blocks * n, per_cu * n_cu_);
+    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, li, kv,
+                       tw_.at(L), a, (int)blocks, (int)(blocks * n));
+  }
+
   // x inverse passes of all work fields formed from the state (modes 2 and 3): one-input and
   // two-input fields in one launch each with cb, else one launch per field
   void x_inv_ops() {
@@ -1326,6 +1451,8 @@ class Engine : public EngineBase {
     a.S = S_;
     a.colloid = colloid_;
     a.alpha = (T)p_.alpha;
+    a.late = ((p_.flags & GNSS_FLAG_UNFUSED) == 0 && x_mode() >= 4) ? 1 : 0;
+    a.kz = tkz_;
     const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
     const double bytes = (double)nx_ * ny_ * (double)kz_lim_ * sizeof(C) * (W_ + F_);
     timed("fft_z_fused", bytes, [&] { GNSS_Z_SWITCH(zfused_launch, a, scale); });
@@ -1353,7 +1480,8 @@ class Engine : public EngineBase {
     const RkCoef c = rk_coef(s);
     const int xm = unfused ? 0 : x_mode();
     const bool xf = xm == 1;                  // group-fused inverse side
-    const bool xfwd = xm == 1 || xm == 3;     // group-fused forward side
+    const bool xfwd = xm == 1 || xm == 3 || xm == 5;  // group-fused forward side
+    const bool late = xm >= 4;                // late-derivative inverse side
     if (xf) {
       x_fused(true, -1, c);
       for (int q = 0; q < S_; ++q) x_fused(true, q, c);
@@ -1372,9 +1500,13 @@ class Engine : public EngineBase {
       });
       for (int f = 0; f < F_; ++f) fwd3d(nlp_[f], nl_[f], true);
     } else {
-      if (xm >= 2) x_inv_ops();
-      else if (xm == 0) col_passes(ws_.data(), W_, ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
-      col_passes(ws_.data(), W_, ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+      if (late) {
+        late_inverse();
+      } else {
+        if (xm >= 2) x_inv_ops();
+        else if (xm == 0) col_passes(ws_.data(), W_, ColSpec{0, +1, kz_lim_ / cx_, g_.keep_y, g_.keep_x, -1});
+        col_passes(ws_.data(), W_, ColSpec{1, +1, kz_lim_ / cy_, -1, g_.keep_y, -1});
+      }
       z_fused();
       col_passes(nl_.data(), F_, ColSpec{1, -1, kz_lim_ / cy_, -1, -1, g_.keep_y});
       if (!xfwd) col_passes(nl_.data(), F_, ColSpec{0, -1, kz_lim_ / cx_, g_.keep_y, -1, g_.keep_x});

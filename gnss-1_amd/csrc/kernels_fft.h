@@ -178,6 +178,29 @@ struct ZRow {
     if (t == 0) r.x[0].y = (active && M < lim) ? src[M].x : (T)0;
   }
 
+  // Row formed on loading: SA A + SB i kz B (SA in {-1, 0, 1}, SB in {-1, 1}), entries k >= lim 0.
+  template <int SA, int SB>
+  __device__ __forceinline__ static void load_ikz(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
+                                                  const T* __restrict__ kz, bool active, int lim, int t, Raw& r) {
+    auto at = [&](unsigned k) {
+      const cx<T> b = B[k];
+      const T kk = (T)SB * kz[k];
+      cx<T> v = mk<T>(-kk * b.y, kk * b.x);
+      if (SA != 0) {
+        const cx<T> x = A[k];
+        v.x += (T)SA * x.x;
+        v.y += (T)SA * x.y;
+      }
+      return v;
+    };
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + e * TPC;
+      r.x[e] = (active && k < lim) ? at((unsigned)k) : mk<T>(0, 0);
+    }
+    if (t == 0) r.x[0].y = (active && M < lim) ? at(M).x : (T)0;
+  }
+
   // Twiddles of one thread for every row it transforms, read once and kept in registers: the
   // radix stages' (TwRegs) and the packing factors twN[k] of its positions k = t + e TPC.  As
   // (E/2) TPC = NZ/4, twN[k] of register e >= E/2 is -i times that of register e - E/2.
@@ -488,6 +511,10 @@ template <typename T> struct ZFusedArgs {
   int kzs, lim, S, G;
   int colloid;           // 1: scalar S - 1 is a colloid drifting with alpha grad(scalar 0)
   T alpha;
+  // late-derivative mode: the rows hold u, v, w, w_y, w_x, omega_z and per scalar phi_x, phi_y, phi;
+  // omega_x = w_y - i kz v, omega_y = i kz u - w_x and phi_z = i kz phi are formed on loading
+  int late;
+  const T* kz;           // kz wavenumber of each stored z mode
 };
 
 template <typename V> __device__ __forceinline__ void fma_pair(V& acc, const V& a, const V& b) {
@@ -533,23 +560,35 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   cx<T> vel[3][E], om[3][E];
 #pragma unroll
   for (int b = 0; b < 3; ++b) Z::load(a.ws[b] + off, active, a.lim, t, r0[b]);
-  if (PREFETCH) {
+  auto load_omega = [&]() {
+    if (a.late) {
+      Z::template load_ikz<1, -1>(a.ws[3] + off, a.ws[1] + off, a.kz, active, a.lim, t, r1[0]);  // w_y - i kz v
+      Z::template load_ikz<-1, 1>(a.ws[4] + off, a.ws[0] + off, a.kz, active, a.lim, t, r1[1]);  // i kz u - w_x
+      Z::load(a.ws[5] + off, active, a.lim, t, r1[2]);
+    } else {
 #pragma unroll
-    for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
-  }
+      for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
+    }
+  };
+  auto load_grad = [&](int s) {
+    const int w0 = 6 + 3 * s;
+    if (a.late) {
+      Z::load(a.ws[w0] + off, active, a.lim, t, r0[0]);
+      Z::load(a.ws[w0 + 1] + off, active, a.lim, t, r0[1]);
+      Z::template load_ikz<0, 1>(a.ws[w0 + 2] + off, a.ws[w0 + 2] + off, a.kz, active, a.lim, t, r0[2]);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Z::load(a.ws[w0 + b] + off, active, a.lim, t, r0[b]);
+    }
+  };
+  if (PREFETCH) load_omega();
   Z::template c2r<3, !DEFER>(r0, lds, idx, t, twM, twN, scale, vel);  // u, v, w
-  if (!PREFETCH) {
-#pragma unroll
-    for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
-  }
+  if (!PREFETCH) load_omega();
   // Plain scalars; with a colloid, scalar 0 (its carrier Z) goes last so that the velocity
   // registers can then take the drift velocity u + alpha grad Z.
   const int P = a.S - a.colloid;
   const int s_first = (a.colloid && P > 1) ? 1 : 0;
-  if (PREFETCH && P > 0) {
-#pragma unroll
-    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s_first + b] + off, active, a.lim, t, r0[b]);
-  }
+  if (PREFETCH && P > 0) load_grad(s_first);
   Z::template c2r<3, !DEFER>(r1, lds, idx, t, twM, twN, scale, om);  // omega
   // u x omega, in place of omega
 #pragma unroll
@@ -566,10 +605,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   }
   for (int j = 0; j < P; ++j) {
     const int s = a.colloid ? (j + 1 < P ? j + 1 : 0) : j;
-    if (!PREFETCH || j > 0) {
-#pragma unroll
-      for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * s + b] + off, active, a.lim, t, r0[b]);
-    }
+    if (!PREFETCH || j > 0) load_grad(s);
     Z::template c2r<3, !DEFER>(r0, lds, idx, t, twM, twN, scale, om);  // grad phi
     cx<T> n[1][E];
 #pragma unroll

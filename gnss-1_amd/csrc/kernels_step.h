@@ -610,6 +610,105 @@ k_xinv_op_pipe(XOpList<T> li, const T* __restrict__ kxv, const T* __restrict__ k
   }
 }
 
+// Inverse strided passes with the derivatives along the transformed axis applied on the way in
+// (the late-derivative x mode): list entry i reads one or two input fields A, B and writes one or
+// two transformed outputs out_j = (a_j + i b_j k) A + (c_j + i d_j k) B with a_j .. d_j in
+// {-1, 0, 1}, k the wavenumber of the element's mode along the axis.  The x pass reads the state
+// once per field (u, v, w, phi) and writes it with its x-derivatives; the y pass adds the
+// y-derivatives and omega_z = v_x - i k_y u; the fused z pass forms the remaining terms with
+// i k_z (replaces the spectral prologue of the reference's computeRHS, solver.cu:419-437, and the
+// curl/gradient kernels it calls).
+struct PairOp {
+  int nin, nout;
+  float a[2], b[2], c[2], d[2];
+};
+template <typename T> struct PairList {
+  const cx<T>* A[MAXW];
+  const cx<T>* B[MAXW];
+  cx<T>* out[2][MAXW];
+  PairOp op[MAXW];
+};
+
+// Persistent and software-pipelined like k_xinv_op_pipe: tile i is tile i % tpf of list entry
+// i / tpf; A of the next tile is loaded after the first output is stored, B when the tile is
+// formed.  An entry may write its outputs over its own inputs (each tile is read before it is
+// written, by the one block that owns it); no entry may read a field another entry writes.
+template <typename T, int L, int C>
+__global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
+k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restrict__ tw, ColArgs a, int tpf,
+                int ntiles) {
+  constexpr int E = Plan<L>::E;
+  constexpr int TPC = L / E;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = (tid / C) % TPC;
+  const int g = tid / (C * TPC);
+  auto offset_of = [&](int tile, bool& active) {
+    const int lt = tile % tpf;
+    const int chunk = lt % a.nchunks;
+    const int o = (lt / a.nchunks) * a.G + g;
+    active = o < a.nouter;
+    const int io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
+    return (long long)io * a.ostride + chunk * C + c;
+  };
+  auto load_col = [&](const cx<T>* __restrict__ src, int tile, cx<T>* va) {
+    bool active;
+    const long long off = offset_of(tile, active);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int p = t + e * TPC;
+      va[e] = (active && row_kept(p, L, a.in_keep)) ? src[off + (long long)p * a.estride] : mk<T>(0, 0);
+    }
+  };
+  auto store_col = [&](cx<T>* __restrict__ dst, int tile, const cx<T>* v) {
+    bool active;
+    const long long off = offset_of(tile, active);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) dst[off + (long long)(t + e * TPC) * a.estride] = v[e];
+    }
+  };
+  const LdsCol idx{g * L * C + c, C};
+  int tile = blockIdx.x;
+  cx<T> va[E];
+  load_col(li.A[tile / tpf], tile, va);
+  while (true) {
+    const int ent = tile / tpf;
+    const PairOp op = li.op[ent];
+    cx<T> vb[E], v0[E], v1[E];
+    if (op.nin == 2) load_col(li.B[ent], tile, vb);
+    else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) vb[e] = mk<T>(0, 0);
+    }
+    auto form = [&](int j, T k, const cx<T>& A, const cx<T>& B) {
+      const T ka = (T)op.b[j] * k, kb = (T)op.d[j] * k, ra = (T)op.a[j], rb = (T)op.c[j];
+      return mk<T>(ra * A.x - ka * A.y + rb * B.x - kb * B.y, ra * A.y + ka * A.x + rb * B.y + kb * B.x);
+    };
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const T k = kv[t + e * TPC];
+      v0[e] = form(0, k, va[e], vb[e]);
+      v1[e] = form(1, k, va[e], vb[e]);
+    }
+    // the next tile's A is loaded while the second output transforms (at most two sets of
+    // columns are live at a time)
+    const int next = tile + gridDim.x;
+    const bool more = next < ntiles;
+    fft<T, L, +1>(v0, lds, t, idx, tw);
+    store_col(li.out[0][ent], tile, v0);
+    if (more) load_col(li.A[next / tpf], next, va);
+    if (op.nout == 2) {
+      fft<T, L, +1>(v1, lds, t, idx, tw);
+      store_col(li.out[1][ent], tile, v1);
+    }
+    if (!more) break;
+    tile = next;
+  }
+}
+
 // Forward x pass of one field group's nonlinear terms fused with the projection (velocity),
 // the sign of the scalar term (computeRHSKernel_mgpu solver.cu:461-521), the mask and the RK3-IF
 // update of the state (replaces solver.cu:547-657): the transformed nonlinear terms never return

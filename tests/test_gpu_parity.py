@@ -189,6 +189,51 @@ def test_per_field_x_inverse_matches(g, engine, prec):
             assert rel(out[1][f], out[0][f]) <= (1e-13 if prec == 0 else 2e-6), (pair, f)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n,sc", [((64, 48, 96), (0.7, 2.0)), ((96, 64, 128), (1.0,)), ((32, 24, 48), ()),
+                                  ((48, 32, 16), (0.5, 1.0, 2.0, 4.0))], ids=lambda v: "x".join(map(str, v)))
+def test_late_derivative_modes_match(g, engine, prec, n, sc):
+    """x modes 4 and 5 apply each derivative in the pass of its own axis (the x pass reads u, v, w,
+    phi once; the y pass forms w_y, phi_y and omega_z = v_x - i k_y u; the fused z pass forms
+    omega_x, omega_y and phi_z with i k_z as it loads).  Same step as modes 2 and 3, which form
+    every work field from the state in the x pass, to rounding (FP64 1e-13, FP32 2e-6); with one
+    launch per pass (cb = 1) and per field, and with each fused z variant."""
+    cfg = g.Config(n=n, re=400.0, sc=sc, theta=0.08, noise_kmax=8.0, noise_amp=0.2, dt=1e-3, precision=prec)
+    nf = 3 + len(sc)
+    zfs = (0, 3, 6, 7) if prec == 0 else (0, 1, 3, 6, 7)
+    print(f"\n[late derivatives n={n} S={len(sc)} prec={prec}]", flush=True)
+    for early, late in (("xf=2", "xf=4"), ("xf=3", "xf=5")):
+        ref = g.Solver(cfg, engine).setup()
+        ref.set_tuning(early + " cb=1")
+        ref.step(3)
+        want = [ref.get_spectral(f) for f in range(nf)]
+        ref.close()
+        for extra in ["cb=1 zf=%d" % z for z in zfs] + ["cb=0"]:
+            a = g.Solver(cfg, engine).setup()
+            a.set_tuning(late + " " + extra)
+            a.step(3)
+            for f in range(nf):
+                assert rel(a.get_spectral(f), want[f]) <= (1e-13 if prec == 0 else 2e-6), (late, extra, f)
+            a.close()
+
+
+def test_late_derivative_mode_matches_oracle(g, engine, oracle):
+    """The 64^3 jet, 100 FP64 steps in x mode 4 against the oracle (rel-L2 <= 1e-12), and the mode
+    is refused for a colloid, whose work fields need second derivatives."""
+    cfg = g.config("jet64")
+    a = g.Solver(cfg, engine).setup()
+    a.set_tuning("xf=4")
+    b = g.Solver(cfg, oracle).setup()
+    a.step(100)
+    b.step(100)
+    for f in range(4):
+        assert rel(a.get_spectral(f), b.get_spectral(f)) <= 1e-12, f
+    c = g.Solver(g.with_colloid(g.config("jet64")), engine).setup()
+    for spec in ("xf=4", "xf=5"):
+        with pytest.raises(g.GnssError):
+            c.set_tuning(spec)
+
+
 def test_fp32_error_vs_fp64(g, engine):
     """FP32 form against the FP64 plain form on the same seed (64^3 jet, 100 steps)."""
     a = g.Solver(g.config("jet64", precision=g.FP32), engine).setup()
