@@ -622,7 +622,6 @@ class Engine : public EngineBase {
           for (int v = 0; v <= max_xmode(); ++v) {
             for (int c : {4, 8, 16}) {
               if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once per cb
-              if (v >= 4 && sizeof(T) == 4 && c != 8) continue;  // FP32 late passes use 8-column tiles
               cb_ = b;
               xf_ = v;
               if (v) xc_ = c;
@@ -1213,6 +1212,7 @@ class Engine : public EngineBase {
   // phi_y and omega_z = v_x - i k_y u; the fused z pass forms omega_x = w_y - i k_z v,
   // omega_y = i k_z u - w_x and phi_z = i k_z phi as it loads.
   void late_inverse() {
+    if (nsp_ >= (1LL << 31)) throw Error(GNSS_ERR_UNSUPPORTED, "late passes index fields with 32-bit offsets");
     const PairOp copy{1, 1, {1, 0}, {0, 0}, {0, 0}, {0, 0}};        // out0 = A
     const PairOp copy_d{1, 2, {1, 0}, {0, 1}, {0, 0}, {0, 0}};      // out0 = A, out1 = i k A
     const PairOp curl_z{2, 2, {1, 0}, {0, -1}, {0, 1}, {0, 0}};     // out0 = A, out1 = B - i k A
@@ -1262,68 +1262,83 @@ class Engine : public EngineBase {
     pair_pass(lx, nxe, ax, nx_, tkx_, "fft_x_inv_late");
     pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late");
   }
-  int pair_chunk() const { return sizeof(T) == 4 ? 8 : xc_; }
+  int pair_chunk() const { return xc_; }
+  // one launch per (inputs, outputs) kind with cb, else one per entry
   void pair_pass(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name) {
     const int CC = pair_chunk();
     auto rows = [&](int keep) { return keep < 0 ? L : std::min(L, 2 * keep + 1); };
     const double col = (double)a.nouter * a.nchunks * CC * sizeof(C);
-    auto bytes_of = [&](int i0, int i1) {
-      double b = 0;
-      for (int i = i0; i < i1; ++i) b += col * (li.op[i].nin * rows(a.in_keep) + li.op[i].nout * rows(a.out_keep));
-      return b;
-    };
-    const int groups = cb_ ? 1 : n, per = cb_ ? n : 1;
-    for (int gi = 0; gi < groups; ++gi) {
-      PairList<T> sub{};
-      for (int i = 0; i < per; ++i) {
-        const int j = gi * per + i;
-        sub.A[i] = li.A[j];
-        sub.B[i] = li.B[j];
-        sub.out[0][i] = li.out[0][j];
-        sub.out[1][i] = li.out[1][j];
-        sub.op[i] = li.op[j];
-      }
-      timed(name, bytes_of(gi * per, gi * per + per), [&] {
-        switch (L) {
-          case 16: pair_c<16>(sub, per, a, kv, CC); break;
-          case 24: pair_c<24>(sub, per, a, kv, CC); break;
-          case 32: pair_c<32>(sub, per, a, kv, CC); break;
-          case 48: pair_c<48>(sub, per, a, kv, CC); break;
-          case 64: pair_c<64>(sub, per, a, kv, CC); break;
-          case 96: pair_c<96>(sub, per, a, kv, CC); break;
-          case 128: pair_c<128>(sub, per, a, kv, CC); break;
-          case 192: pair_c<192>(sub, per, a, kv, CC); break;
-          case 256: pair_c<256>(sub, per, a, kv, CC); break;
-          case 384: pair_c<384>(sub, per, a, kv, CC); break;
-          case 512: pair_c<512>(sub, per, a, kv, CC); break;
-          case 1024: pair_c<1024>(sub, per, a, kv, CC); break;
-          default: throw Error(GNSS_ERR_UNSUPPORTED, "late pass length " + std::to_string(L));
+    for (int kind = 0; kind < 3; ++kind) {
+      const int nin = kind == 2 ? 2 : 1, nout = kind == 0 ? 1 : 2;
+      std::vector<int> ents;
+      for (int i = 0; i < n; ++i)
+        if (li.op[i].nin == nin && li.op[i].nout == nout) ents.push_back(i);
+      const int groups = cb_ ? (ents.empty() ? 0 : 1) : (int)ents.size();
+      for (int gi = 0; gi < groups; ++gi) {
+        const int i0 = cb_ ? 0 : gi, i1 = cb_ ? (int)ents.size() : gi + 1;
+        PairList<T> sub{};
+        for (int i = i0; i < i1; ++i) {
+          const int j = ents[i], k = i - i0;
+          sub.A[k] = li.A[j];
+          sub.B[k] = li.B[j];
+          sub.out[0][k] = li.out[0][j];
+          sub.out[1][k] = li.out[1][j];
+          sub.op[k] = li.op[j];
         }
-      });
+        const int m = i1 - i0;
+        timed(name, col * m * (nin * rows(a.in_keep) + nout * rows(a.out_keep)), [&] {
+          switch (L) {
+            case 16: pair_c<16>(sub, m, a, kv, CC, kind); break;
+            case 24: pair_c<24>(sub, m, a, kv, CC, kind); break;
+            case 32: pair_c<32>(sub, m, a, kv, CC, kind); break;
+            case 48: pair_c<48>(sub, m, a, kv, CC, kind); break;
+            case 64: pair_c<64>(sub, m, a, kv, CC, kind); break;
+            case 96: pair_c<96>(sub, m, a, kv, CC, kind); break;
+            case 128: pair_c<128>(sub, m, a, kv, CC, kind); break;
+            case 192: pair_c<192>(sub, m, a, kv, CC, kind); break;
+            case 256: pair_c<256>(sub, m, a, kv, CC, kind); break;
+            case 384: pair_c<384>(sub, m, a, kv, CC, kind); break;
+            case 512: pair_c<512>(sub, m, a, kv, CC, kind); break;
+            case 1024: pair_c<1024>(sub, m, a, kv, CC, kind); break;
+            default: throw Error(GNSS_ERR_UNSUPPORTED, "late pass length " + std::to_string(L));
+          }
+        });
+      }
     }
   }
   template <int L>
-  void pair_c(const PairList<T>& li, int n, ColArgs a, const T* kv, int CC) {
+  void pair_c(const PairList<T>& li, int n, const ColArgs& a, const T* kv, int CC, int kind) {
     if constexpr (sizeof(T) == 8) {
-      if (CC == 8) pair_launch<L, 8>(li, n, a, kv);
-      else pair_launch<L, 4>(li, n, a, kv);
+      if (CC == 8) pair_k<L, 8>(li, n, a, kv, kind);
+      else pair_k<L, 4>(li, n, a, kv, kind);
     } else {
-      pair_launch<L, 8>(li, n, a, kv);
+      if (CC == 16) pair_k<L, 16>(li, n, a, kv, kind);
+      else pair_k<L, 8>(li, n, a, kv, kind);
     }
   }
   template <int L, int CC>
+  void pair_k(const PairList<T>& li, int n, const ColArgs& a, const T* kv, int kind) {
+    if (kind == 0) pair_launch<L, CC, 1, 1>(li, n, a, kv);
+    else if (kind == 1) pair_launch<L, CC, 1, 2>(li, n, a, kv);
+    else pair_launch<L, CC, 2, 2>(li, n, a, kv);
+  }
+  template <int L, int CC, int NIN, int NOUT>
   void pair_launch(const PairList<T>& li, int n, ColArgs a, const T* kv) {
     constexpr int TPC = L / Plan<L>::E;
+    // two outputs use two LDS regions: up to the 160 KiB a gfx950 workgroup can hold
+    constexpr long long PAIR_LDS_MAX = 160 * 1024;
+    const long long region = (long long)NOUT * L * CC * (long long)sizeof(C);
     int G = std::max(1, 256 / (CC * TPC));
-    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
+    while (G > 1 && G * region > PAIR_LDS_MAX) --G;
+    if (region > PAIR_LDS_MAX) throw Error(GNSS_ERR_UNSUPPORTED, "late pass tile does not fit the LDS");
     a.G = G;
     const int threads = CC * TPC * G;
-    const size_t lds = (size_t)G * L * CC * sizeof(C);
+    const size_t lds = (size_t)(G * region);
     const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
     const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds, 2048 / threads));
     const long long grid = std::min<long long>(blocks * n, per_cu * n_cu_);
-    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, li, kv,
-                       tw_.at(L), a, (int)blocks, (int)(blocks * n));
+    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC, NIN, NOUT>), dim3((unsigned)grid), dim3(threads), lds, s_, li,
+                       kv, tw_.at(L), a, (int)blocks, (int)(blocks * n));
   }
 
   // x inverse passes of all work fields formed from the state (modes 2 and 3): one-input and

@@ -629,11 +629,13 @@ template <typename T> struct PairList {
   PairOp op[MAXW];
 };
 
-// Persistent and software-pipelined like k_xinv_op_pipe: tile i is tile i % tpf of list entry
-// i / tpf; A of the next tile is loaded after the first output is stored, B when the tile is
-// formed.  An entry may write its outputs over its own inputs (each tile is read before it is
-// written, by the one block that owns it); no entry may read a field another entry writes.
-template <typename T, int L, int C>
+// Persistent like k_xinv_op_pipe: tile i is tile i % tpf of list entry i / tpf; B is loaded when
+// the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch (two outputs go
+// through the radix stages together, in two LDS regions).  Element offsets are 32-bit: a field
+// holds fewer than 2^31 elements (checked by the host).  An entry may write its outputs over its own inputs (each tile is read
+// before it is written, by the one block that owns it); no entry may read a field another entry
+// writes.
+template <typename T, int L, int C, int NIN, int NOUT>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
 k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restrict__ tw, ColArgs a, int tpf,
                 int ntiles) {
@@ -651,58 +653,63 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     const int o = (lt / a.nchunks) * a.G + g;
     active = o < a.nouter;
     const int io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
-    return (long long)io * a.ostride + chunk * C + c;
+    return (unsigned)io * (unsigned)a.ostride + (unsigned)(chunk * C + c);
   };
   auto load_col = [&](const cx<T>* __restrict__ src, int tile, cx<T>* va) {
     bool active;
-    const long long off = offset_of(tile, active);
+    const unsigned off = offset_of(tile, active);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int p = t + e * TPC;
-      va[e] = (active && row_kept(p, L, a.in_keep)) ? src[off + (long long)p * a.estride] : mk<T>(0, 0);
+      va[e] = (active && row_kept(p, L, a.in_keep)) ? src[off + (unsigned)p * (unsigned)a.estride] : mk<T>(0, 0);
     }
   };
-  auto store_col = [&](cx<T>* __restrict__ dst, int tile, const cx<T>* v) {
-    bool active;
-    const long long off = offset_of(tile, active);
-    if (active) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) dst[off + (long long)(t + e * TPC) * a.estride] = v[e];
-    }
-  };
-  const LdsCol idx{g * L * C + c, C};
+  // G groups of C columns; with two outputs each group owns two regions of L x C entries
+  const LdsCol idx{g * NOUT * L * C + c, C};
+  // one output: the next tile's A is loaded before the current tile transforms; two outputs:
+  // A is loaded with the tile (three sets of columns would not fit the registers)
+  constexpr bool PF = NOUT == 1;
   int tile = blockIdx.x;
   cx<T> va[E];
-  load_col(li.A[tile / tpf], tile, va);
+  if (PF) load_col(li.A[tile / tpf], tile, va);
   while (true) {
     const int ent = tile / tpf;
     const PairOp op = li.op[ent];
-    cx<T> vb[E], v0[E], v1[E];
-    if (op.nin == 2) load_col(li.B[ent], tile, vb);
-    else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) vb[e] = mk<T>(0, 0);
-    }
-    auto form = [&](int j, T k, const cx<T>& A, const cx<T>& B) {
-      const T ka = (T)op.b[j] * k, kb = (T)op.d[j] * k, ra = (T)op.a[j], rb = (T)op.c[j];
-      return mk<T>(ra * A.x - ka * A.y + rb * B.x - kb * B.y, ra * A.y + ka * A.x + rb * B.y + kb * B.x);
-    };
+    if (!PF) load_col(li.A[ent], tile, va);
+    cx<T> vb[NIN == 2 ? E : 1], v[NOUT][E];
+    if (NIN == 2) load_col(li.B[ent], tile, vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const T k = kv[t + e * TPC];
-      v0[e] = form(0, k, va[e], vb[e]);
-      v1[e] = form(1, k, va[e], vb[e]);
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) {
+        const T ka = (T)op.b[j] * k, ra = (T)op.a[j];
+        cx<T> r = mk<T>(ra * va[e].x - ka * va[e].y, ra * va[e].y + ka * va[e].x);
+        if (NIN == 2) {
+          const T kb = (T)op.d[j] * k, rb = (T)op.c[j];
+          const cx<T> b = vb[NIN == 2 ? e : 0];
+          r.x += rb * b.x - kb * b.y;
+          r.y += rb * b.y + kb * b.x;
+        }
+        v[j][e] = r;
+      }
     }
-    // the next tile's A is loaded while the second output transforms (at most two sets of
-    // columns are live at a time)
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
-    fft<T, L, +1>(v0, lds, t, idx, tw);
-    store_col(li.out[0][ent], tile, v0);
-    if (more) load_col(li.A[next / tpf], next, va);
-    if (op.nout == 2) {
-      fft<T, L, +1>(v1, lds, t, idx, tw);
-      store_col(li.out[1][ent], tile, v1);
+    if (PF && more) load_col(li.A[next / tpf], next, va);
+    // the twiddles are re-read for each tile rather than held in registers across the tile loop
+    const cx<T>* twl = tw;
+    asm volatile("" : "+s"(twl));
+    fft_batch<T, L, +1, LdsCol, Plan<L>, false, NOUT, L * C>(v, lds, t, idx, twl);
+    bool active;
+    const unsigned off = offset_of(tile, active);
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) {
+        cx<T>* __restrict__ dst = li.out[j][ent];
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride] = v[j][e];
+      }
     }
     if (!more) break;
     tile = next;
