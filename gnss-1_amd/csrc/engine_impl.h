@@ -415,13 +415,13 @@ class Engine : public EngineBase {
     return b;
   }
   std::string tuning_string() const {
-    char b[64];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d cb=%d", cx_, cy_, zf_, xf_, cp_, xc_, zp_,
-             kp_, cb_);
+    char b[96];
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d cb=%d lp=%d", cx_, cy_, zf_, xf_, cp_, xc_,
+             zp_, kp_, cb_, lp_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_, cb = cb_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_, cb = cb_, lp = lp_;
     const char* p = spec.c_str();
     const char* q;
     if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
@@ -433,8 +433,10 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "zp=")) || (q = strstr(p, "\"zp\": "))) zp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "kp=")) || (q = strstr(p, "\"kp\": "))) kp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cb=")) || (q = strstr(p, "\"cb\": "))) cb = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "lp=")) || (q = strstr(p, "\"lp\": "))) lp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > max_xmode() ||
-        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1)
+        cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1 || lp < 0 ||
+        lp > 1)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
@@ -447,6 +449,7 @@ class Engine : public EngineBase {
     zp_ = zp;
     kp_ = kp;
     cb_ = cb;
+    lp_ = lp;
   }
 
   // -------------------------------------------------------------------------------------------
@@ -616,26 +619,31 @@ class Engine : public EngineBase {
           GNSS_HIP(hipMemcpyAsync(grab(sizeof(C) * nsp_), dq_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
         }
         // x-pass mode and tile, jointly with one launch per pass (cb = 1) or per field (cb = 0)
-        int best = xf_, best_xc = xc_, best_cb = cb_;
+        int best = xf_, best_xc = xc_, best_cb = cb_, best_lp = lp_;
         double best_ms = 1e30;
         for (int b = 0; b <= 1; ++b) {
           for (int v = 0; v <= max_xmode(); ++v) {
             for (int c : {4, 8, 16}) {
               if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once per cb
-              cb_ = b;
-              xf_ = v;
-              if (v) xc_ = c;
-              const double ms = time_it([&] { substage(1); });
-              if (v) snprintf(buf, sizeof buf, ", \"cb=%d xf=%d xc=%d\": %.5f", b, v, c, ms);
-              else snprintf(buf, sizeof buf, ", \"cb=%d xf=0\": %.5f", b, ms);
-              rep += buf;
-              if (ms < best_ms) { best_ms = ms; best = v; best_xc = xc_; best_cb = b; }
+              for (int l = (v >= 4 ? 0 : 1); l <= 1; ++l) {   // lp matters in the late modes only
+                cb_ = b;
+                xf_ = v;
+                lp_ = l;
+                if (v) xc_ = c;
+                const double ms = time_it([&] { substage(1); });
+                if (v >= 4) snprintf(buf, sizeof buf, ", \"cb=%d xf=%d xc=%d lp=%d\": %.5f", b, v, c, l, ms);
+                else if (v) snprintf(buf, sizeof buf, ", \"cb=%d xf=%d xc=%d\": %.5f", b, v, c, ms);
+                else snprintf(buf, sizeof buf, ", \"cb=%d xf=0\": %.5f", b, ms);
+                rep += buf;
+                if (ms < best_ms) { best_ms = ms; best = v; best_xc = xc_; best_cb = b; best_lp = l; }
+              }
             }
           }
         }
         xf_ = best;
         xc_ = best_xc;
         cb_ = best_cb;
+        lp_ = best_lp;
         for (int f = 0; f < F_; ++f) {
           GNSS_HIP(hipMemcpyAsync(q_[f], snap[2 * f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
           GNSS_HIP(hipMemcpyAsync(dq_[f], snap[2 * f + 1], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
@@ -1263,13 +1271,41 @@ class Engine : public EngineBase {
     pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late");
   }
   int pair_chunk() const { return xc_; }
-  // one launch per (inputs, outputs) kind with cb, else one per entry
+  // With lp = 1 an entry's two outputs go through one launch; with lp = 0 they are split into
+  // single-output entries that read the inputs again, the second outputs first (they read fields
+  // the first outputs overwrite in place).  Then one launch per (inputs, outputs) kind with cb,
+  // else one per entry.
   void pair_pass(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name) {
+    if (lp_) return pair_kinds(li, n, a, L, kv, name);
+    PairList<T> first{}, second{};
+    int n1 = 0, n2 = 0;
+    for (int i = 0; i < n; ++i) {
+      const PairOp& op = li.op[i];
+      const bool first_uses_b = op.c[0] != 0 || op.d[0] != 0;
+      const PairOp o0{first_uses_b ? 2 : 1, 1, {op.a[0], 0}, {op.b[0], 0}, {op.c[0], 0}, {op.d[0], 0}};
+      first.A[n1] = li.A[i];
+      first.B[n1] = li.B[i];
+      first.out[0][n1] = first.out[1][n1] = li.out[0][i];
+      first.op[n1++] = o0;
+      if (op.nout == 2) {
+        const bool uses_b = op.c[1] != 0 || op.d[1] != 0;
+        PairOp o1{uses_b ? 2 : 1, 1, {op.a[1], 0}, {op.b[1], 0}, {op.c[1], 0}, {op.d[1], 0}};
+        second.A[n2] = li.A[i];
+        second.B[n2] = li.B[i];
+        second.out[0][n2] = second.out[1][n2] = li.out[1][i];
+        second.op[n2++] = o1;
+      }
+    }
+    pair_kinds(second, n2, a, L, kv, name);
+    pair_kinds(first, n1, a, L, kv, name);
+  }
+  void pair_kinds(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name) {
     const int CC = pair_chunk();
     auto rows = [&](int keep) { return keep < 0 ? L : std::min(L, 2 * keep + 1); };
     const double col = (double)a.nouter * a.nchunks * CC * sizeof(C);
-    for (int kind = 0; kind < 3; ++kind) {
-      const int nin = kind == 2 ? 2 : 1, nout = kind == 0 ? 1 : 2;
+    static const int KIN[4] = {1, 1, 2, 2}, KOUT[4] = {1, 2, 2, 1};
+    for (int kind = 0; kind < 4; ++kind) {
+      const int nin = KIN[kind], nout = KOUT[kind];
       std::vector<int> ents;
       for (int i = 0; i < n; ++i)
         if (li.op[i].nin == nin && li.op[i].nout == nout) ents.push_back(i);
@@ -1320,7 +1356,8 @@ class Engine : public EngineBase {
   void pair_k(const PairList<T>& li, int n, const ColArgs& a, const T* kv, int kind) {
     if (kind == 0) pair_launch<L, CC, 1, 1>(li, n, a, kv);
     else if (kind == 1) pair_launch<L, CC, 1, 2>(li, n, a, kv);
-    else pair_launch<L, CC, 2, 2>(li, n, a, kv);
+    else if (kind == 2) pair_launch<L, CC, 2, 2>(li, n, a, kv);
+    else pair_launch<L, CC, 2, 1>(li, n, a, kv);
   }
   template <int L, int CC, int NIN, int NOUT>
   void pair_launch(const PairList<T>& li, int n, ColArgs a, const T* kv) {
@@ -1542,7 +1579,7 @@ class Engine : public EngineBase {
   gnss_params p_;
   int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
-  int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, cb_ = 0, n_cu_ = 256;
+  int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, cb_ = 0, lp_ = 1, n_cu_ = 256;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

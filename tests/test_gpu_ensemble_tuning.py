@@ -82,6 +82,7 @@ def test_autotune_reports_and_applies(g, engine):
     assert "cb=0 xf=0" in rep["candidates"] and "cp=1" in rep["candidates"] and "zp=1" in rep["candidates"]
     assert {"cb=0 xf=2 xc=8", "cb=1 xf=2 xc=16", "cb=0 xf=3 xc=8", "cb=1 xf=3 xc=16", "cb=1 xf=0"} <= set(rep["candidates"])
     assert {"cb=0 xf=1 xc=8", "cb=1 xf=1 xc=16"} <= set(rep["candidates"])
+    assert {"cb=1 xf=4 xc=16 lp=0", "cb=1 xf=4 xc=16 lp=1", "cb=0 xf=5 xc=8 lp=1"} <= set(rep["candidates"])
     assert s.autotune(0)["choice"] == rep["choice"]
     with pytest.raises(g.GnssError):
         s.set_tuning("cx=3")
