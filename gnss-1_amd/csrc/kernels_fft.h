@@ -178,13 +178,15 @@ struct ZRow {
     if (t == 0) r.x[0].y = (active && M < lim) ? src[M].x : (T)0;
   }
 
-  // Row formed on loading: SA A + SB i kz B (SA in {-1, 0, 1}, SB in {-1, 1}), entries k >= lim 0.
+  // Row formed on loading: SA A + SB i kz B (SA in {-1, 0, 1}, SB in {-1, 1}), entries k >= lim 0;
+  // kzr[e]: kz of this thread's position k = t + e TPC, kz: the table (for k = M).
   template <int SA, int SB>
   __device__ __forceinline__ static void load_ikz(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
-                                                  const T* __restrict__ kz, bool active, int lim, int t, Raw& r) {
-    auto at = [&](unsigned k) {
+                                                  const T* kzr, const T* __restrict__ kz, bool active, int lim,
+                                                  int t, Raw& r) {
+    auto at = [&](unsigned k, T kzk) {
       const cx<T> b = B[k];
-      const T kk = (T)SB * kz[k];
+      const T kk = (T)SB * kzk;
       cx<T> v = mk<T>(-kk * b.y, kk * b.x);
       if (SA != 0) {
         const cx<T> x = A[k];
@@ -196,9 +198,9 @@ struct ZRow {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + e * TPC;
-      r.x[e] = (active && k < lim) ? at((unsigned)k) : mk<T>(0, 0);
+      r.x[e] = (active && k < lim) ? at((unsigned)k, kzr[e]) : mk<T>(0, 0);
     }
-    if (t == 0) r.x[0].y = (active && M < lim) ? at(M).x : (T)0;
+    if (t == 0) r.x[0].y = (active && M < lim) ? at(M, kz[M]).x : (T)0;
   }
 
   // Twiddles of one thread for every row it transforms, read once and kept in registers: the
@@ -560,10 +562,14 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   cx<T> vel[3][E], om[3][E];
 #pragma unroll
   for (int b = 0; b < 3; ++b) Z::load(a.ws[b] + off, active, a.lim, t, r0[b]);
+  // late mode: kz of this thread's positions, read once per row
+  T kzr[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) kzr[e] = (a.late && t + e * Z::TPC < a.lim) ? a.kz[t + e * Z::TPC] : (T)0;
   auto load_omega = [&]() {
     if (a.late) {
-      Z::template load_ikz<1, -1>(a.ws[3] + off, a.ws[1] + off, a.kz, active, a.lim, t, r1[0]);  // w_y - i kz v
-      Z::template load_ikz<-1, 1>(a.ws[4] + off, a.ws[0] + off, a.kz, active, a.lim, t, r1[1]);  // i kz u - w_x
+      Z::template load_ikz<1, -1>(a.ws[3] + off, a.ws[1] + off, kzr, a.kz, active, a.lim, t, r1[0]);  // w_y - i kz v
+      Z::template load_ikz<-1, 1>(a.ws[4] + off, a.ws[0] + off, kzr, a.kz, active, a.lim, t, r1[1]);  // i kz u - w_x
       Z::load(a.ws[5] + off, active, a.lim, t, r1[2]);
     } else {
 #pragma unroll
@@ -575,7 +581,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
     if (a.late) {
       Z::load(a.ws[w0] + off, active, a.lim, t, r0[0]);
       Z::load(a.ws[w0 + 1] + off, active, a.lim, t, r0[1]);
-      Z::template load_ikz<0, 1>(a.ws[w0 + 2] + off, a.ws[w0 + 2] + off, a.kz, active, a.lim, t, r0[2]);
+      Z::template load_ikz<0, 1>(a.ws[w0 + 2] + off, a.ws[w0 + 2] + off, kzr, a.kz, active, a.lim, t, r0[2]);
     } else {
 #pragma unroll
       for (int b = 0; b < 3; ++b) Z::load(a.ws[w0 + b] + off, active, a.lim, t, r0[b]);

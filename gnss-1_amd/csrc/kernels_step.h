@@ -697,9 +697,10 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
     if (PF && more) load_col(li.A[next / tpf], next, va);
-    // the twiddles are re-read for each tile rather than held in registers across the tile loop
+    // with two outputs the twiddles are re-read for each tile rather than held in registers
+    // across the tile loop (the registers are needed for the second output)
     const cx<T>* twl = tw;
-    asm volatile("" : "+s"(twl));
+    if (NOUT == 2) asm volatile("" : "+s"(twl));
     fft_batch<T, L, +1, LdsCol, Plan<L>, false, NOUT, L * C>(v, lds, t, idx, twl);
     bool active;
     const unsigned off = offset_of(tile, active);
