@@ -347,10 +347,11 @@ class Engine : public EngineBase {
   void bench_copy(int reps, double* ms, double* bytes) override {
     if (reps <= 0) throw Error(GNSS_ERR_ARG, "reps must be > 0");
     const long long n16 = (long long)(sizeof(C) * nsp_ / 16);
-    const long long grid = std::min<long long>(cdiv(n16, 256 * 4), (long long)n_cu_ * 64);
+    const long long chunk = 4096;  // 64 KiB per block
+    const long long grid = cdiv(n16, chunk);
     auto launch = [&] {
-      hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, s_, reinterpret_cast<const float4*>(ws_[0]),
-                         reinterpret_cast<float4*>(ws_[1]), n16);
+      hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, s_, reinterpret_cast<const v4f*>(ws_[0]),
+                         reinterpret_cast<v4f*>(ws_[1]), n16, chunk);
     };
     hipEvent_t e0, e1;
     GNSS_HIP(hipEventCreate(&e0));

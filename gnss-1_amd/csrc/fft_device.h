@@ -30,6 +30,42 @@ template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
 }
 template <typename V> __device__ __forceinline__ V cconj(V a) { a.y = -a.y; return a; }
 template <typename V, typename S> __device__ __forceinline__ V cscale(V a, S s) { a.x *= s; a.y *= s; return a; }
+// HBM element I/O of the transform passes.  A pass streams a whole field through and the next
+// pass reads it only after every block has finished, so nothing is gained by keeping the lines in
+// L2 or the last-level cache: with GNSS_NT_STORE / GNSS_NT_LOAD the element goes through a
+// nontemporal (streaming) vector store / load.
+#ifndef GNSS_NT_STORE
+#define GNSS_NT_STORE 1
+#endif
+#ifndef GNSS_NT_LOAD
+#define GNSS_NT_LOAD 0
+#endif
+template <typename V> __device__ __forceinline__ void st_s(V* p, const V& v) {
+#if GNSS_NT_STORE
+  using E = decltype(v.x);
+  typedef E v2 __attribute__((ext_vector_type(2)));
+  v2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+#else
+  *p = v;
+#endif
+}
+template <typename V> __device__ __forceinline__ V ld_s(const V* p) {
+#if GNSS_NT_LOAD
+  using E = decltype(p->x);
+  typedef E v2 __attribute__((ext_vector_type(2)));
+  const v2 w = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  V r;
+  r.x = w.x;
+  r.y = w.y;
+  return r;
+#else
+  return *p;
+#endif
+}
+
 // multiply by (SIGN * i)
 template <int SIGN, typename V> __device__ __forceinline__ V cmul_si(V a) {
   V r;

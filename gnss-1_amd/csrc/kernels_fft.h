@@ -73,7 +73,7 @@ k_col(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int p = t + e * TPC;
-    if (active && row_kept(p, L, a.in_keep)) v[e] = base[(long long)p * a.estride];
+    if (active && row_kept(p, L, a.in_keep)) v[e] = ld_s(&base[(long long)p * a.estride]);
     else v[e] = mk<T>(0, 0);
   }
   fft<T, L, SIGN>(v, lds, t, LdsCol{g * L * C + c, C}, tw);
@@ -81,7 +81,7 @@ k_col(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int p = t + e * TPC;
-    if (row_kept(p, L, a.out_keep)) base[(long long)p * a.estride] = v[e];
+    if (row_kept(p, L, a.out_keep)) st_s(&base[(long long)p * a.estride], v[e]);
   }
 }
 
@@ -119,7 +119,7 @@ k_col_multi(FieldList<T> fl, const cx<T>* __restrict__ tw, ColArgs a, int tpf, i
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int p = t + e * TPC;
-      v[e] = (active && row_kept(p, L, a.in_keep)) ? base[(long long)p * a.estride] : mk<T>(0, 0);
+      v[e] = (active && row_kept(p, L, a.in_keep)) ? ld_s(&base[(long long)p * a.estride]) : mk<T>(0, 0);
     }
   };
   int tile = blockIdx.x;
@@ -137,7 +137,7 @@ k_col_multi(FieldList<T> fl, const cx<T>* __restrict__ tw, ColArgs a, int tpf, i
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int p = t + e * TPC;
-        if (row_kept(p, L, a.out_keep)) base[(long long)p * a.estride] = v[e];
+        if (row_kept(p, L, a.out_keep)) st_s(&base[(long long)p * a.estride], v[e]);
       }
     }
     if (!more) break;
@@ -173,7 +173,7 @@ struct ZRow {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + e * TPC;
-      r.x[e] = (active && k < lim) ? src[(unsigned)k] : mk<T>(0, 0);
+      r.x[e] = (active && k < lim) ? ld_s(&src[(unsigned)k]) : mk<T>(0, 0);
     }
     if (t == 0) r.x[0].y = (active && M < lim) ? src[M].x : (T)0;
   }
@@ -185,11 +185,11 @@ struct ZRow {
                                                   const T* kzr, const T* __restrict__ kz, bool active, int lim,
                                                   int t, Raw& r) {
     auto at = [&](unsigned k, T kzk) {
-      const cx<T> b = B[k];
+      const cx<T> b = ld_s(&B[k]);
       const T kk = (T)SB * kzk;
       cx<T> v = mk<T>(-kk * b.y, kk * b.x);
       if (SA != 0) {
-        const cx<T> x = A[k];
+        const cx<T> x = ld_s(&A[k]);
         v.x += (T)SA * x.x;
         v.y += (T)SA * x.y;
       }
@@ -313,11 +313,11 @@ struct ZRow {
             cx<T> x;
             x.x = h * (zk.x + c.x + p.y);
             x.y = h * (zk.y + c.y - p.x);
-            dst[b][(unsigned)k] = x;
+            st_s(&dst[b][(unsigned)k], x);
           }
         }
-        if (t0 && M < lim) dst[b][M] = mk<T>(((T)2 * h) * (v[b][0].x - v[b][0].y), (T)0);
-        for (int k = M + 1 + t; k < kzs && k < lim; k += TPC) dst[b][k] = mk<T>(0, 0);
+        if (t0 && M < lim) st_s(&dst[b][M], mk<T>(((T)2 * h) * (v[b][0].x - v[b][0].y), (T)0));
+        for (int k = M + 1 + t; k < kzs && k < lim; k += TPC) st_s(&dst[b][k], mk<T>(0, 0));
       }
     }
   }
@@ -339,7 +339,7 @@ k_r2c_z(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restri
   const cx<T>* src = reinterpret_cast<const cx<T>*>(in + (active ? row : 0) * NZ);
   cx<T> v[1][Z::E];
 #pragma unroll
-  for (int e = 0; e < Z::E; ++e) v[0][e] = active ? src[t + e * Z::TPC] : mk<T>(0, 0);
+  for (int e = 0; e < Z::E; ++e) v[0][e] = active ? ld_s(&src[t + e * Z::TPC]) : mk<T>(0, 0);
   cx<T>* dst[1] = {out + (active ? row : 0) * kzs};
   Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, twM, twN);
 }
@@ -364,7 +364,7 @@ k_c2r_z(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __restri
   if (!active) return;
   cx<T>* dst = reinterpret_cast<cx<T>*>(out + row * NZ);
 #pragma unroll
-  for (int e = 0; e < Z::E; ++e) dst[t + e * Z::TPC] = v[0][e];
+  for (int e = 0; e < Z::E; ++e) st_s(&dst[t + e * Z::TPC], v[0][e]);
 }
 
 // Persistent, software-pipelined forms of k_r2c_z / k_c2r_z: each block walks row groups
@@ -385,7 +385,7 @@ k_r2c_z_pipe(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __r
     const bool active = row < nrows;
     const cx<T>* src = reinterpret_cast<const cx<T>*>(in + (active ? row : 0) * NZ);
 #pragma unroll
-    for (int e = 0; e < Z::E; ++e) v[e] = active ? src[t + e * Z::TPC] : mk<T>(0, 0);
+    for (int e = 0; e < Z::E; ++e) v[e] = active ? ld_s(&src[t + e * Z::TPC]) : mk<T>(0, 0);
   };
   long long grp = blockIdx.x;
   cx<T> v[1][Z::E], nv[Z::E];
@@ -435,7 +435,7 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
     if (active) {
       cx<T>* dst = reinterpret_cast<cx<T>*>(out + row * NZ);
 #pragma unroll
-      for (int e = 0; e < Z::E; ++e) dst[t + e * Z::TPC] = v[0][e];
+      for (int e = 0; e < Z::E; ++e) st_s(&dst[t + e * Z::TPC], v[0][e]);
     }
     if (!more) break;
     r[0] = rn;

@@ -127,8 +127,8 @@ __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const 
   const int mz = mode_of(iz, g.nz);
   if (iz >= g.nzh || !kept(g, mode_of(ix, g.nx), mode_of(iy, g.ny), mz)) {
     for (int f = 0; f < F; ++f) {
-      q.p[f][o] = mk<T>(0, 0);
-      if (!c.last) dq.p[f][o] = mk<T>(0, 0);
+      st_s(&q.p[f][o], mk<T>(0, 0));
+      if (!c.last) st_s(&dq.p[f][o], mk<T>(0, 0));
     }
     return;
   }
@@ -139,7 +139,7 @@ __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const 
 #pragma unroll
     for (int f = 0; f < MAXF; ++f) rhs[f] = mk<T>(0, 0);
   } else {
-    const cx<T> n0 = nl.p[0][o], n1 = nl.p[1][o], n2 = nl.p[2][o];
+    const cx<T> n0 = ld_s(&nl.p[0][o]), n1 = ld_s(&nl.p[1][o]), n2 = ld_s(&nl.p[2][o]);
     cx<T> kn;
     kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
     kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
@@ -148,7 +148,7 @@ __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const 
     rhs[2] = csub(n2, cscale(kn, kz));
 #pragma unroll
     for (int f = 3; f < MAXF; ++f) {
-      const cx<T> v = f < F ? nl.p[f][o] : mk<T>(0, 0);
+      const cx<T> v = f < F ? ld_s(&nl.p[f][o]) : mk<T>(0, 0);
       rhs[f] = mk<T>(-v.x, -v.y);
     }
   }
@@ -159,15 +159,15 @@ __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const 
     const T e = dexp<T>(-(T)c.nu[f] * k2 * (T)c.tau);
     cx<T> d = cscale(rhs[f], dt);
     if (!c.first) {
-      const cx<T> old = dq.p[f][o];
+      const cx<T> old = ld_s(&dq.p[f][o]);
       d.x = a * old.x + d.x;
       d.y = a * old.y + d.y;
     }
-    cx<T> qq = q.p[f][o];
+    cx<T> qq = ld_s(&q.p[f][o]);
     qq.x = e * (qq.x + bb * d.x);
     qq.y = e * (qq.y + bb * d.y);
-    q.p[f][o] = qq;
-    if (!c.last) dq.p[f][o] = cscale(d, e);
+    st_s(&q.p[f][o], qq);
+    if (!c.last) st_s(&dq.p[f][o], cscale(d, e));
   }
 }
 
@@ -477,7 +477,7 @@ k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, cons
 #pragma unroll
     for (int f = 0; f < NIN; ++f) {
       const int fi = f == 0 ? in0 : (GRP == 2 ? 3 : in0 + f);  // the colloid group also reads Z
-      qv[f][e] = ld ? q.p[fi][col + p * estride] : mk<T>(0, 0);
+      qv[f][e] = ld ? ld_s(&q.p[fi][col + p * estride]) : mk<T>(0, 0);
     }
   }
   const LdsCol idx{gg * L * C + c, C};
@@ -506,7 +506,7 @@ k_xinv_prologue(Geom g, CPtrs<T> q, CPtrs<T> ws, const T* __restrict__ kxv, cons
     fft<T, L, +1>(v, lds, t, idx, tw);
     if (active) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) ws.p[out0 + w][col + (long long)(t + e * TPC) * estride] = v[e];
+      for (int e = 0; e < E; ++e) st_s(&ws.p[out0 + w][col + (long long)(t + e * TPC) * estride], v[e]);
     }
   }
 }
@@ -562,7 +562,7 @@ k_xinv_op_pipe(XOpList<T> li, const T* __restrict__ kxv, const T* __restrict__ k
     for (int e = 0; e < E; ++e) {
       const int p = t + e * TPC;
       const bool ld = active && row_kept(p, L, a.in_keep);
-      va[e] = ld ? src[off + (long long)p * a.estride] : mk<T>(0, 0);
+      va[e] = ld ? ld_s(&src[off + (long long)p * a.estride]) : mk<T>(0, 0);
     }
   };
   auto form = [&](int tile, const cx<T>* va, cx<T>* v) {
@@ -603,7 +603,7 @@ k_xinv_op_pipe(XOpList<T> li, const T* __restrict__ kxv, const T* __restrict__ k
     if (active) {
       cx<T>* __restrict__ out = li.out[tile / tpf];
 #pragma unroll
-      for (int e = 0; e < E; ++e) out[off + (long long)(t + e * TPC) * a.estride] = v[e];
+      for (int e = 0; e < E; ++e) st_s(&out[off + (long long)(t + e * TPC) * a.estride], v[e]);
     }
     if (!more) break;
     tile = next;
@@ -661,7 +661,7 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int p = t + e * TPC;
-      va[e] = (active && row_kept(p, L, a.in_keep)) ? src[off + (unsigned)p * (unsigned)a.estride] : mk<T>(0, 0);
+      va[e] = (active && row_kept(p, L, a.in_keep)) ? ld_s(&src[off + (unsigned)p * (unsigned)a.estride]) : mk<T>(0, 0);
     }
   };
   // G groups of C columns; with two outputs each group owns two regions of L x C entries
@@ -709,7 +709,7 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
       for (int j = 0; j < NOUT; ++j) {
         cx<T>* __restrict__ dst = li.out[j][ent];
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride] = v[j][e];
+        for (int e = 0; e < E; ++e) st_s(&dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride], v[j][e]);
       }
     }
     if (!more) break;
@@ -750,7 +750,7 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   for (int f = 0; f < NF; ++f) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
-      nv[f][e] = active ? nl.p[f0 + f][col + (long long)(t + e * TPC) * estride] : mk<T>(0, 0);
+      nv[f][e] = active ? ld_s(&nl.p[f0 + f][col + (long long)(t + e * TPC) * estride]) : mk<T>(0, 0);
     if (f > 0) __syncthreads();
     fft<T, L, -1>(nv[f], lds, t, idx, tw);
   }
@@ -767,8 +767,8 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
     if (iz >= g.nzh || !kept(g, mode_of(p, g.nx), my, mz)) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        q.p[f0 + f][o2] = mk<T>(0, 0);
-        if (!rc.last) dq.p[f0 + f][o2] = mk<T>(0, 0);
+        st_s(&q.p[f0 + f][o2], mk<T>(0, 0));
+        if (!rc.last) st_s(&dq.p[f0 + f][o2], mk<T>(0, 0));
       }
       continue;
     }
@@ -796,33 +796,37 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
     for (int f = 0; f < NF; ++f) {
       cx<T> d = cscale(rhs[f], dt);
       if (!rc.first) {
-        const cx<T> old = dq.p[f0 + f][o2];
+        const cx<T> old = ld_s(&dq.p[f0 + f][o2]);
         d.x = a * old.x + d.x;
         d.y = a * old.y + d.y;
       }
-      cx<T> qq = q.p[f0 + f][o2];
+      cx<T> qq = ld_s(&q.p[f0 + f][o2]);
       qq.x = ef * (qq.x + bb * d.x);
       qq.y = ef * (qq.y + bb * d.y);
-      q.p[f0 + f][o2] = qq;
-      if (!rc.last) dq.p[f0 + f][o2] = cscale(d, ef);
+      st_s(&q.p[f0 + f][o2], qq);
+      if (!rc.last) st_s(&dq.p[f0 + f][o2], cscale(d, ef));
     }
   }
 }
 
-// Plain copy (bandwidth reference): grid-stride, four 16-byte loads in flight per thread before
-// the stores.
-static __global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ in, float4* __restrict__ out,
-                                                       long long n) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    float4 a[4];
+// Plain copy (bandwidth reference): block b copies the contiguous chunk [b chunk, (b + 1) chunk)
+// of 16-byte elements with four nontemporal loads in flight per thread before the stores.  Of the
+// copy forms measured on MI355X (grid-stride or chunked, 2-8 loads in flight, cached or
+// nontemporal) this one moves the most bytes per second.
+typedef float v4f __attribute__((ext_vector_type(4)));
+static __global__ void __launch_bounds__(256) k_copy16(const v4f* __restrict__ in, v4f* __restrict__ out,
+                                                       long long n, long long chunk) {
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = beg + chunk < n ? beg + chunk : n;
+  long long i = beg + threadIdx.x;
+  for (; i + 3 * 256 < end; i += 4 * 256) {
+    v4f a[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = in[i + u * stride];
+    for (int u = 0; u < 4; ++u) a[u] = __builtin_nontemporal_load(&in[i + u * 256]);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) out[i + u * stride] = a[u];
+    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(a[u], &out[i + u * 256]);
   }
-  for (; i < n; i += stride) out[i] = in[i];
+  for (; i < end; i += 256) __builtin_nontemporal_store(__builtin_nontemporal_load(&in[i]), &out[i]);
 }
 
 // ---------------------------------------------------------------------------------------------
