@@ -437,7 +437,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "lp=")) || (q = strstr(p, "\"lp\": "))) lp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > max_xmode() ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1 || lp < 0 ||
-        lp > 1)
+        lp > 2)
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
@@ -626,7 +626,7 @@ class Engine : public EngineBase {
           for (int v = 0; v <= max_xmode(); ++v) {
             for (int c : {4, 8, 16}) {
               if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once per cb
-              for (int l = (v >= 4 ? 0 : 1); l <= 1; ++l) {   // lp matters in the late modes only
+              for (int l = (v >= 4 ? 0 : 1); l <= (v >= 4 ? 2 : 1); ++l) {  // lp: late modes only
                 cb_ = b;
                 xf_ = v;
                 lp_ = l;
@@ -1272,7 +1272,8 @@ class Engine : public EngineBase {
     pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late");
   }
   int pair_chunk() const { return xc_; }
-  // With lp = 1 an entry's two outputs go through one launch; with lp = 0 they are split into
+  // With lp = 1 an entry's two outputs go through one launch (lp = 2: transformed one after the
+  // other through one LDS region, which halves the kernel's LDS); with lp = 0 they are split into
   // single-output entries that read the inputs again, the second outputs first (they read fields
   // the first outputs overwrite in place).  Then one launch per (inputs, outputs) kind with cb,
   // else one per entry.
@@ -1355,17 +1356,19 @@ class Engine : public EngineBase {
   }
   template <int L, int CC>
   void pair_k(const PairList<T>& li, int n, const ColArgs& a, const T* kv, int kind) {
-    if (kind == 0) pair_launch<L, CC, 1, 1>(li, n, a, kv);
-    else if (kind == 1) pair_launch<L, CC, 1, 2>(li, n, a, kv);
-    else if (kind == 2) pair_launch<L, CC, 2, 2>(li, n, a, kv);
-    else pair_launch<L, CC, 2, 1>(li, n, a, kv);
+    if (kind == 0) pair_launch<L, CC, 1, 1, false>(li, n, a, kv);
+    else if (kind == 1 && lp_ == 2) pair_launch<L, CC, 1, 2, true>(li, n, a, kv);
+    else if (kind == 1) pair_launch<L, CC, 1, 2, false>(li, n, a, kv);
+    else if (kind == 2 && lp_ == 2) pair_launch<L, CC, 2, 2, true>(li, n, a, kv);
+    else if (kind == 2) pair_launch<L, CC, 2, 2, false>(li, n, a, kv);
+    else pair_launch<L, CC, 2, 1, false>(li, n, a, kv);
   }
-  template <int L, int CC, int NIN, int NOUT>
+  template <int L, int CC, int NIN, int NOUT, bool SEQ>
   void pair_launch(const PairList<T>& li, int n, ColArgs a, const T* kv) {
     constexpr int TPC = L / Plan<L>::E;
-    // two outputs use two LDS regions: up to the 160 KiB a gfx950 workgroup can hold
+    // two batched outputs use two LDS regions: up to the 160 KiB a gfx950 workgroup can hold
     constexpr long long PAIR_LDS_MAX = 160 * 1024;
-    const long long region = (long long)NOUT * L * CC * (long long)sizeof(C);
+    const long long region = (long long)(SEQ ? 1 : NOUT) * L * CC * (long long)sizeof(C);
     int G = std::max(1, 256 / (CC * TPC));
     while (G > 1 && G * region > PAIR_LDS_MAX) --G;
     if (region > PAIR_LDS_MAX) throw Error(GNSS_ERR_UNSUPPORTED, "late pass tile does not fit the LDS");
@@ -1375,7 +1378,7 @@ class Engine : public EngineBase {
     const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
     const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds, 2048 / threads));
     const long long grid = std::min<long long>(blocks * n, per_cu * n_cu_);
-    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC, NIN, NOUT>), dim3((unsigned)grid), dim3(threads), lds, s_, li,
+    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC, NIN, NOUT, SEQ>), dim3((unsigned)grid), dim3(threads), lds, s_, li,
                        kv, tw_.at(L), a, (int)blocks, (int)(blocks * n));
   }
 

@@ -630,12 +630,14 @@ template <typename T> struct PairList {
 };
 
 // Persistent like k_xinv_op_pipe: tile i is tile i % tpf of list entry i / tpf; B is loaded when
-// the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch (two outputs go
-// through the radix stages together, in two LDS regions).  Element offsets are 32-bit: a field
-// holds fewer than 2^31 elements (checked by the host).  An entry may write its outputs over its own inputs (each tile is read
-// before it is written, by the one block that owns it); no entry may read a field another entry
-// writes.
-template <typename T, int L, int C, int NIN, int NOUT>
+// the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch.  Two outputs go
+// through the radix stages together, in two LDS regions (SEQ = false), or one after the other
+// through one region (SEQ = true: half the LDS, and the next tile's A is loaded once the last
+// output is formed, before it transforms).  Element offsets are 32-bit: a field holds fewer than
+// 2^31 elements (checked by the host).  An entry may write its outputs over its own inputs (each
+// tile is read before it is written, by the one block that owns it, and both outputs are formed
+// from registers); no entry may read a field another entry writes.
+template <typename T, int L, int C, int NIN, int NOUT, bool SEQ>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
 k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restrict__ tw, ColArgs a, int tpf,
                 int ntiles) {
@@ -664,11 +666,28 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
       va[e] = (active && row_kept(p, L, a.in_keep)) ? ld_s(&src[off + (unsigned)p * (unsigned)a.estride]) : mk<T>(0, 0);
     }
   };
-  // G groups of C columns; with two outputs each group owns two regions of L x C entries
-  const LdsCol idx{g * NOUT * L * C + c, C};
-  // one output: the next tile's A is loaded before the current tile transforms; two outputs:
-  // A is loaded with the tile (three sets of columns would not fit the registers)
-  constexpr bool PF = NOUT == 1;
+  // G groups of C columns; each group owns NREG regions of L x C entries
+  constexpr int NREG = SEQ ? 1 : NOUT;
+  const LdsCol idx{g * NREG * L * C + c, C};
+  // the next tile's A is loaded before the current tile transforms with one output or SEQ; with
+  // two outputs batched A is loaded with the tile (three sets of columns would not fit the registers)
+  constexpr bool PF = NOUT == 1 || SEQ;
+  // output j of entry op at this thread's positions
+  auto form = [&](const PairOp& op, int j, const cx<T>* va, const cx<T>* vb, cx<T>* r) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const T k = kv[t + e * TPC];
+      const T ka = (T)op.b[j] * k, ra = (T)op.a[j];
+      cx<T> x = mk<T>(ra * va[e].x - ka * va[e].y, ra * va[e].y + ka * va[e].x);
+      if (NIN == 2) {
+        const T kb = (T)op.d[j] * k, rb = (T)op.c[j];
+        const cx<T> b = vb[NIN == 2 ? e : 0];
+        x.x += rb * b.x - kb * b.y;
+        x.y += rb * b.y + kb * b.x;
+      }
+      r[e] = x;
+    }
+  };
   int tile = blockIdx.x;
   cx<T> va[E];
   if (PF) load_col(li.A[tile / tpf], tile, va);
@@ -676,40 +695,42 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     const int ent = tile / tpf;
     const PairOp op = li.op[ent];
     if (!PF) load_col(li.A[ent], tile, va);
-    cx<T> vb[NIN == 2 ? E : 1], v[NOUT][E];
+    cx<T> vb[NIN == 2 ? E : 1];
     if (NIN == 2) load_col(li.B[ent], tile, vb);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const T k = kv[t + e * TPC];
-#pragma unroll
-      for (int j = 0; j < NOUT; ++j) {
-        const T ka = (T)op.b[j] * k, ra = (T)op.a[j];
-        cx<T> r = mk<T>(ra * va[e].x - ka * va[e].y, ra * va[e].y + ka * va[e].x);
-        if (NIN == 2) {
-          const T kb = (T)op.d[j] * k, rb = (T)op.c[j];
-          const cx<T> b = vb[NIN == 2 ? e : 0];
-          r.x += rb * b.x - kb * b.y;
-          r.y += rb * b.y + kb * b.x;
-        }
-        v[j][e] = r;
-      }
-    }
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
-    if (PF && more) load_col(li.A[next / tpf], next, va);
     // with two outputs the twiddles are re-read for each tile rather than held in registers
     // across the tile loop (the registers are needed for the second output)
     const cx<T>* twl = tw;
     if (NOUT == 2) asm volatile("" : "+s"(twl));
-    fft_batch<T, L, +1, LdsCol, Plan<L>, false, NOUT, L * C>(v, lds, t, idx, twl);
     bool active;
     const unsigned off = offset_of(tile, active);
-    if (active) {
+    if constexpr (SEQ) {
 #pragma unroll
       for (int j = 0; j < NOUT; ++j) {
-        cx<T>* __restrict__ dst = li.out[j][ent];
+        cx<T> v[1][E];
+        form(op, j, va, vb, v[0]);
+        if (j == NOUT - 1 && more) load_col(li.A[next / tpf], next, va);
+        fft_batch<T, L, +1, LdsCol, Plan<L>, false, 1, L * C>(v, lds, t, idx, twl);
+        if (active) {
+          cx<T>* __restrict__ dst = li.out[j][ent];
 #pragma unroll
-        for (int e = 0; e < E; ++e) st_s(&dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride], v[j][e]);
+          for (int e = 0; e < E; ++e) st_s(&dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride], v[0][e]);
+        }
+      }
+    } else {
+      cx<T> v[NOUT][E];
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) form(op, j, va, vb, v[j]);
+      if (PF && more) load_col(li.A[next / tpf], next, va);
+      fft_batch<T, L, +1, LdsCol, Plan<L>, false, NOUT, L * C>(v, lds, t, idx, twl);
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) {
+          cx<T>* __restrict__ dst = li.out[j][ent];
+#pragma unroll
+          for (int e = 0; e < E; ++e) st_s(&dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride], v[j][e]);
+        }
       }
     }
     if (!more) break;

@@ -14,7 +14,7 @@ def test_table_is_valid():
         prec, nx, ny, nz, s = key.split()
         assert prec in ("f32", "f64") and int(s.rstrip("c")) >= (2 if s.endswith("c") else 0)
         assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp", "kp", "cb", "lp"}
-        assert v.get("cb", 0) in (0, 1) and v.get("lp", 1) in (0, 1)
+        assert v.get("cb", 0) in (0, 1) and v.get("lp", 1) in (0, 1, 2)
         assert v.get("kp", 0) in range(5)
         allowed = {4, 8} if prec == "f64" else {8, 16}
         assert v["cx"] in allowed and v["cy"] in allowed and v.get("xc", v["cx"]) in allowed
