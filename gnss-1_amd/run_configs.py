@@ -7,8 +7,11 @@
             radix-16 stage; errors against each other and against FP64 at 512^3 (same physical
             parameters) compared on the 512^3 modes
   colloid512  the 512^3 jet plus the colloid scalar, FP32 and FP64, 100 steps
+  jet512    the headline 512^3 jet, FP32 and FP64 forms on the same seed, 100 steps: the FP32
+            form's errors against FP64 (fields, integral statistics, profiles, spectrum) and its
+            divergence, against the precision form's bounds (1e-5, 1e-4, 1e-6)
 
-usage: python gnss-1_amd/run_configs.py {jet256,ensemble,jet1024,colloid512} [--steps K]
+usage: python gnss-1_amd/run_configs.py {jet256,ensemble,jet1024,colloid512,jet512} [--steps K]
 """
 import argparse
 import json
@@ -142,14 +145,51 @@ def run_colloid512(steps):
     return out
 
 
+def run_jet512(steps):
+    """FP32 form against the FP64 plain form, 512^3 jet, same seed, `steps` RK3 steps.  Reductions
+    (statistics, profiles, spectrum) are FP64 in both forms; errors are relative (L2 for fields
+    and profiles, absolute value over the FP64 value for scalar statistics)."""
+    out = {"workload": "jet512 fp32 vs fp64", "steps": steps}
+    res = {}
+    for name, prec in (("fp32", gnss.FP32), ("fp64", gnss.FP64)):
+        s = gnss.Solver(gnss.config("jet512", precision=prec)).setup()
+        t0 = time.perf_counter()
+        s.step(steps)
+        dt = time.perf_counter() - t0
+        out[f"{name}_steps_per_s"] = steps / dt
+        st, yp, sp = s.stats()
+        res[name] = (st, yp, sp, [s.get_spectral(f) for f in range(4)])
+        log(f"[jet512] {name}: {steps / dt:.2f} steps/s")
+        s.close()
+    (s32, y32, p32, f32), (s64, y64, p64, f64) = res["fp32"], res["fp64"]
+    out["rel_l2"] = {k: rel(f32[i], f64[i]) for i, k in enumerate(("u", "v", "w", "phi0"))}
+    keys = ("ke", "vrms", "epsilon", "enstrophy", "cfl", "umax", "lint", "lambda", "eta")
+    out["stats_rel"] = {k: abs(s32[k] - s64[k]) / abs(s64[k]) for k in keys if k in s64 and s64[k] != 0}
+    out["stats_rel"]["chi0"] = abs(s32["chi"][0] - s64["chi"][0]) / abs(s64["chi"][0])
+    out["stats_rel"]["scalar_var0"] = abs(s32["scalar_var"][0] - s64["scalar_var"][0]) / abs(s64["scalar_var"][0])
+    # profile rows that vanish in exact arithmetic (the mean of v: d<v>/dy = 0 for a divergence-free
+    # field, and its volume mean is 0) hold only roundoff in FP64: they are reported by their
+    # largest absolute FP32 value instead
+    big = [r for r in range(y64.shape[0]) if np.max(np.abs(y64[r])) > 1e-8]
+    out["profiles_rel_l2_max"] = max(rel(y32[r], y64[r]) for r in big)
+    out["profiles_zero_rows_absmax"] = max((float(np.max(np.abs(y32[r]))) for r in range(y64.shape[0])
+                                            if r not in big), default=0.0)
+    out["spectrum_rel_l2"] = rel(p32, p64)
+    out["divergence_fp32"] = s32["divergence"]
+    out["divergence_fp64"] = s64["divergence"]
+    out["within_bounds"] = bool(max(out["rel_l2"].values()) <= 1e-5 and max(out["stats_rel"].values()) <= 1e-4
+                                and out["divergence_fp32"] <= 1e-6)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["jet256", "ensemble", "jet1024", "colloid512"])
+    ap.add_argument("which", choices=["jet256", "ensemble", "jet1024", "colloid512", "jet512"])
     ap.add_argument("--steps", type=int, default=0)
     a = ap.parse_args()
-    steps = a.steps or {"jet256": 200, "ensemble": 200, "jet1024": 50, "colloid512": 100}[a.which]
+    steps = a.steps or {"jet256": 200, "ensemble": 200, "jet1024": 50, "colloid512": 100, "jet512": 100}[a.which]
     fn = {"jet256": run_jet256, "ensemble": run_ensemble, "jet1024": run_jet1024,
-          "colloid512": run_colloid512}[a.which]
+          "colloid512": run_colloid512, "jet512": run_jet512}[a.which]
     print(json.dumps(fn(steps)))
 
 
