@@ -87,6 +87,11 @@ class EngineBase {
   virtual void enqueue_rk3_step() = 0;
   virtual void advance_clock(int nsteps) = 0;
   virtual void check_finite() = 0;
+  // Batched contexts (one engine holding several members of one grid, stepped by shared
+  // launches): the number of members, and the member that the per-member calls (init_jet,
+  // stats, fields, diagnostics) address.
+  virtual int members() const { return 1; }
+  virtual void select_member(int m) = 0;
   std::string err;
 };
 
@@ -106,5 +111,9 @@ std::string library_dir();
 
 EngineBase* make_engine_f64(const gnss_params& p);
 EngineBase* make_engine_f32(const gnss_params& p);
+// nb members of one grid (same sizes, scalars, precision, flags, box and dt; their own Re, Sc,
+// theta and seed) in one batched engine.
+EngineBase* make_engine_batch_f64(const gnss_params* p, int nb);
+EngineBase* make_engine_batch_f32(const gnss_params* p, int nb);
 
 }  // namespace gnss

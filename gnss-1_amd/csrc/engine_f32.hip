@@ -3,4 +3,5 @@
 
 namespace gnss {
 EngineBase* make_engine_f32(const gnss_params& p) { return new Engine<float>(p); }
+EngineBase* make_engine_batch_f32(const gnss_params* p, int nb) { return new Engine<float>(p[0], nb, p); }
 }  // namespace gnss

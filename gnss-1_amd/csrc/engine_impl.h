@@ -32,7 +32,21 @@ class Engine : public EngineBase {
   static constexpr int LDS_MAX = 64 * 1024;
 
  public:
-  explicit Engine(const gnss_params& p) : p_(p) {
+  // nb > 1: a batched context of the members mp[0..nb) of one grid (see make_engine_batch_*)
+  explicit Engine(const gnss_params& p, int nb = 1, const gnss_params* mp = nullptr) : p_(p) {
+    if (nb < 1 || (nb > 1 && !mp)) throw Error(GNSS_ERR_ARG, "a batched context needs its members");
+    nb_ = nb;
+    mp_.assign(1, p);
+    if (nb > 1) {
+      mp_.assign(mp, mp + nb);
+      const unsigned bad = GNSS_FLAG_COLLOID | GNSS_FLAG_MFMA_R16 | GNSS_FLAG_UNFUSED;
+      for (const gnss_params& q : mp_)
+        if (q.nx != p.nx || q.ny != p.ny || q.nz != p.nz || q.n_scalars != p.n_scalars ||
+            q.precision != p.precision || q.flags != p.flags || (q.flags & bad) || q.lx != p.lx ||
+            q.ly != p.ly || q.lz != p.lz || q.dt != p.dt || !(q.re > 0))
+          throw Error(GNSS_ERR_ARG, "batched members must share grid, scalars, precision, flags, box and dt "
+                                    "(no colloid, MFMA or unfused step)");
+    }
     nx_ = p.nx; ny_ = p.ny; nz_ = p.nz;
     nzh_ = nz_ / 2 + 1;
     S_ = p.n_scalars;
@@ -67,10 +81,34 @@ class Engine : public EngineBase {
     }
     try {
       allocate();
+      qb_ = q_;
+      dqb_ = dq_;
+      if (nb_ > 1) {
+        // viscosity and diffusivities of every member for the batched RK3 update
+        std::vector<double> nu((size_t)nb_ * MAXF, 0.0);
+        for (int m = 0; m < nb_; ++m) {
+          for (int f = 0; f < 3; ++f) nu[(size_t)m * MAXF + f] = 1.0 / mp_[m].re;
+          for (int q = 0; q < S_; ++q) nu[(size_t)m * MAXF + 3 + q] = 1.0 / (mp_[m].re * mp_[m].sc[q]);
+        }
+        dalloc(d_num_, sizeof(double) * nu.size());
+        GNSS_HIP(hipMemcpy(d_num_, nu.data(), sizeof(double) * nu.size(), hipMemcpyHostToDevice));
+      }
     } catch (...) {
       release();
       throw;
     }
+  }
+
+  int members() const override { return nb_; }
+  // Per-member calls address member m: its state and RK register, and its parameters.
+  void select_member(int m) override {
+    if (m < 0 || m >= nb_) throw Error(GNSS_ERR_ARG, "member out of range");
+    if (nb_ == 1) return;
+    for (int f = 0; f < F_; ++f) {
+      q_[f] = qb_[f] + (long long)m * nsp_;
+      dq_[f] = dqb_[f] + (long long)m * nsp_;
+    }
+    p_ = mp_[m];
   }
 
   ~Engine() override { release(); }
@@ -123,6 +161,16 @@ class Engine : public EngineBase {
   }
 
   void init_jet() override {
+    for (int m = 0; m < nb_; ++m) {
+      if (nb_ > 1) select_member(m);
+      init_jet_member();
+    }
+    if (nb_ > 1) select_member(0);
+    t_ = 0;
+    step_ = 0;
+    have_ic_ = true;
+  }
+  void init_jet_member() {
     need_k();
     if (!(p_.theta > 0)) throw Error(GNSS_ERR_ARG, "theta must be > 0");
     std::vector<HostIcEntry> h;
@@ -147,9 +195,6 @@ class Engine : public EngineBase {
       check_launch("k_ic_scatter");
     }
     sync();
-    t_ = 0;
-    step_ = 0;
-    have_ic_ = true;
   }
 
   void forward(int f) override {
@@ -178,9 +223,12 @@ class Engine : public EngineBase {
     if (nsteps == 0) return;
     // the step starts from a dealiased state (deAlias at the top of solver_ps, solver.cu:704);
     // every substage keeps it dealiased afterwards
-    timed("k_dealias", (double)nsp_ * F_ * sizeof(C), [&] {
-      hipLaunchKernelGGL(k_dealias<T>, dim3(cdiv(nsp_, 256)), dim3(256), 0, s_, g_, ptrs(q_), F_, nsp_);
-    });
+    for (int m = 0; m < nb_; ++m) {
+      if (nb_ > 1) select_member(m);
+      timed("k_dealias", (double)nsp_ * F_ * sizeof(C), [&] {
+        hipLaunchKernelGGL(k_dealias<T>, dim3(cdiv(nsp_, 256)), dim3(256), 0, s_, g_, ptrs(q_), F_, nsp_);
+      });
+    }
     for (int n = 0; n < nsteps; ++n) enqueue_rk3_step();
     advance_clock(nsteps);
     check_finite();
@@ -190,7 +238,16 @@ class Engine : public EngineBase {
 
   void enqueue_rk3_step() override {
     need_k();
-    for (int s = 0; s < 3; ++s) substage(s);
+    // a batched context steps all its members with the same launches, from member 0's base
+    if (nb_ > 1) select_member(0);
+    bm_ = nb_;
+    try {
+      for (int s = 0; s < 3; ++s) substage(s);
+    } catch (...) {
+      bm_ = 1;
+      throw;
+    }
+    bm_ = 1;
   }
 
   void advance_clock(int nsteps) override {
@@ -201,9 +258,13 @@ class Engine : public EngineBase {
   // NaN/Inf watchdog on the kz = 0 column of every field.
   void check_finite() override {
     GNSS_HIP(hipMemsetAsync(d_flag_, 0, sizeof(int), s_));
-    hipLaunchKernelGGL(k_finite<T>, dim3(cdiv((long long)nx_ * ny_, 256)), dim3(256), 0, s_, g_,
-                       ptrs(q_), F_, d_flag_);
-    check_launch("k_finite");
+    for (int m = 0; m < nb_; ++m) {
+      if (nb_ > 1) select_member(m);
+      hipLaunchKernelGGL(k_finite<T>, dim3(cdiv((long long)nx_ * ny_, 256)), dim3(256), 0, s_, g_,
+                         ptrs(q_), F_, d_flag_);
+      check_launch("k_finite");
+    }
+    if (nb_ > 1) select_member(0);
     int flag = 0;
     GNSS_HIP(hipMemcpyAsync(&flag, d_flag_, sizeof(int), hipMemcpyDeviceToHost, s_));
     sync();
@@ -512,6 +573,7 @@ class Engine : public EngineBase {
 
   std::string autotune(int reps) override {
     need_k();
+    if (nb_ > 1) throw Error(GNSS_ERR_UNSUPPORTED, "autotune a single-member context of the grid (the table is shared)");
     if (reps <= 0) return "{\"key\": \"" + tuning_key() + "\", \"choice\": \"" + tuning_string() + "\"}";
     const bool was_on = timer_.on();
     timer_.enable(false, s_);
@@ -766,21 +828,24 @@ class Engine : public EngineBase {
   void allocate() {
     q_.assign(F_, nullptr); dq_.assign(F_, nullptr); nl_.assign(F_, nullptr);
     phys_.assign(F_, nullptr); ws_.assign(W_, nullptr);
+    // step arrays hold every member of a batched context back to back; physical buffers serve one
+    // member at a time (statistics, transforms, I/O)
+    const long long nst = nsp_ * nb_;
     for (int f = 0; f < F_; ++f) {
-      dalloc(q_[f], sizeof(C) * nsp_);
-      dalloc(dq_[f], sizeof(C) * nsp_);
-      dalloc(nl_[f], sizeof(C) * nsp_);
+      dalloc(q_[f], sizeof(C) * nst);
+      dalloc(dq_[f], sizeof(C) * nst);
+      dalloc(nl_[f], sizeof(C) * nst);
       dalloc(phys_[f], sizeof(T) * nph_);
     }
-    for (int w = 0; w < W_; ++w) dalloc(ws_[w], sizeof(C) * nsp_);
+    for (int w = 0; w < W_; ++w) dalloc(ws_[w], sizeof(C) * nst);
     for (void* v : allocs_) GNSS_HIP(hipMemsetAsync(v, 0, 1, s_));
     for (int f = 0; f < F_; ++f) {
-      GNSS_HIP(hipMemsetAsync(q_[f], 0, sizeof(C) * nsp_, s_));
-      GNSS_HIP(hipMemsetAsync(dq_[f], 0, sizeof(C) * nsp_, s_));
-      GNSS_HIP(hipMemsetAsync(nl_[f], 0, sizeof(C) * nsp_, s_));
+      GNSS_HIP(hipMemsetAsync(q_[f], 0, sizeof(C) * nst, s_));
+      GNSS_HIP(hipMemsetAsync(dq_[f], 0, sizeof(C) * nst, s_));
+      GNSS_HIP(hipMemsetAsync(nl_[f], 0, sizeof(C) * nst, s_));
       GNSS_HIP(hipMemsetAsync(phys_[f], 0, sizeof(T) * nph_, s_));
     }
-    for (int w = 0; w < W_; ++w) GNSS_HIP(hipMemsetAsync(ws_[w], 0, sizeof(C) * nsp_, s_));
+    for (int w = 0; w < W_; ++w) GNSS_HIP(hipMemsetAsync(ws_[w], 0, sizeof(C) * nst, s_));
     dalloc(dkx_, sizeof(double) * nx_);
     dalloc(dky_, sizeof(double) * ny_);
     dalloc(dkz_, sizeof(double) * nzh_);
@@ -978,6 +1043,7 @@ class Engine : public EngineBase {
     a.nchunks = sp.nchunks;
     a.in_keep = sp.in_keep;
     a.out_keep = sp.out_keep;
+    batch_outer(a);
     int G = std::max(1, 256 / (CC * TPC));
     while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
     a.G = G;
@@ -1155,7 +1221,7 @@ class Engine : public EngineBase {
   // prologue and the RK3 update per field group; 2 per-field x inverse passes that form their
   // work field on the fly (forward side as in mode 0); 3 the inverse side of mode 2 with the
   // forward side of mode 1.  The MFMA radix-16 path uses mode 0.
-  int x_mode() const { return (p_.flags & GNSS_FLAG_MFMA_R16) ? 0 : xf_; }
+  int x_mode() const { return (p_.flags & GNSS_FLAG_MFMA_R16) ? 0 : (nb_ > 1 ? 4 : xf_); }
   // modes 4 and 5: late derivatives (the x pass reads each state field once, the y and z passes
   // apply their own derivatives; see late_inverse), with mode 2's and mode 3's forward side.
   // Not built for the colloid, whose work fields need second derivatives.
@@ -1221,7 +1287,7 @@ class Engine : public EngineBase {
   // phi_y and omega_z = v_x - i k_y u; the fused z pass forms omega_x = w_y - i k_z v,
   // omega_y = i k_z u - w_x and phi_z = i k_z phi as it loads.
   void late_inverse() {
-    if (nsp_ >= (1LL << 31)) throw Error(GNSS_ERR_UNSUPPORTED, "late passes index fields with 32-bit offsets");
+    if (nsp_ * bm_ >= (1LL << 31)) throw Error(GNSS_ERR_UNSUPPORTED, "late passes index fields with 32-bit offsets");
     const PairOp copy{1, 1, {1, 0}, {0, 0}, {0, 0}, {0, 0}};        // out0 = A
     const PairOp copy_d{1, 2, {1, 0}, {0, 1}, {0, 0}, {0, 0}};      // out0 = A, out1 = i k A
     const PairOp curl_z{2, 2, {1, 0}, {0, -1}, {0, 1}, {0, 0}};     // out0 = A, out1 = B - i k A
@@ -1268,10 +1334,19 @@ class Engine : public EngineBase {
     ay.in_keep = g_.keep_y;
     ay.out_keep = -1;
     ay.G = 1;
+    batch_outer(ax);
+    batch_outer(ay);
     pair_pass(lx, nxe, ax, nx_, tkx_, "fft_x_inv_late");
     pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late");
   }
   int pair_chunk() const { return xc_; }
+  // A pass of a batched step covers the outer indices of every member.
+  void batch_outer(ColArgs& a) const {
+    if (bm_ <= 1) return;
+    a.nouter_m = a.nouter;
+    a.nouter *= bm_;
+    a.mstride = nsp_;
+  }
   // With lp = 1 an entry's two outputs go through one launch (lp = 2: transformed one after the
   // other through one LDS region, which halves the kernel's LDS); with lp = 0 they are split into
   // single-output entries that read the inputs again, the second outputs first (they read fields
@@ -1501,7 +1576,7 @@ class Engine : public EngineBase {
     ZFusedArgs<T> a{};
     for (int w = 0; w < W_; ++w) a.ws[w] = ws_[w];
     for (int f = 0; f < F_; ++f) a.nl[f] = nl_[f];
-    a.nrows = (long long)nx_ * ny_;
+    a.nrows = (long long)bm_ * nx_ * ny_;  // members lie back to back: their rows continue
     a.kzs = kzs_;
     a.lim = kz_lim_;
     a.S = S_;
@@ -1510,7 +1585,7 @@ class Engine : public EngineBase {
     a.late = ((p_.flags & GNSS_FLAG_UNFUSED) == 0 && x_mode() >= 4) ? 1 : 0;
     a.kz = tkz_;
     const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
-    const double bytes = (double)nx_ * ny_ * (double)kz_lim_ * sizeof(C) * (W_ + F_);
+    const double bytes = (double)bm_ * nx_ * ny_ * (double)kz_lim_ * sizeof(C) * (W_ + F_);
     timed("fft_z_fused", bytes, [&] { GNSS_Z_SWITCH(zfused_launch, a, scale); });
   }
 
@@ -1526,6 +1601,9 @@ class Engine : public EngineBase {
     for (int f = 0; f < MAXF; ++f) c.nu[f] = 0;
     for (int f = 0; f < 3; ++f) c.nu[f] = nu;
     for (int q = 0; q < S_; ++q) c.nu[3 + q] = nu / p_.sc[q];
+    c.nbox = 0;
+    c.mstride = 0;
+    c.nu_m = nullptr;
     return c;
   }
 
@@ -1572,10 +1650,17 @@ class Engine : public EngineBase {
       for (int q = 0; q < S_; ++q) x_fused(false, q, c);
       return;
     }
-    const double rk_bytes = (double)nb * F_ * sizeof(C) * (c.first ? 3 : (c.last ? 4 : 5));
+    RkCoef cm = c;
+    if (bm_ > 1) {
+      cm.nbox = nb;
+      cm.mstride = nsp_;
+      cm.nu_m = d_num_;
+    }
+    const long long n = nb * bm_;
+    const double rk_bytes = (double)n * F_ * sizeof(C) * (c.first ? 3 : (c.last ? 4 : 5));
     timed("k_rk", rk_bytes, [&] {
-      hipLaunchKernelGGL(k_rk<T>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s_, g_, b, ptrs(q_), ptrs(dq_),
-                         ptrs(nl_), tkx_, tky_, tkz_, c, F_, nb);
+      hipLaunchKernelGGL(k_rk<T>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s_, g_, b, ptrs(q_), ptrs(dq_),
+                         ptrs(nl_), tkx_, tky_, tkz_, cm, F_, n);
     });
   }
 
@@ -1584,6 +1669,12 @@ class Engine : public EngineBase {
   int colloid_ = 0;
   int nx_ = 0, ny_ = 0, nz_ = 0, nzh_ = 0, kzs_ = 0, S_ = 0, F_ = 0, W_ = 0, cx_ = 8, cy_ = 8, zf_ = 0, xf_ = 1;
   int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, cb_ = 0, lp_ = 1, n_cu_ = 256;
+  // batched contexts: members held (nb_), members the launches in flight cover (bm_: nb_ inside a
+  // step, 1 elsewhere), member parameters, base pointers of the state, per-member viscosities
+  int nb_ = 1, bm_ = 1;
+  std::vector<gnss_params> mp_;
+  std::vector<C*> qb_, dqb_;
+  double* d_num_ = nullptr;
   int kz_lim_ = 0;
   long long nsp_ = 0, nph_ = 0;
   Geom g_{};

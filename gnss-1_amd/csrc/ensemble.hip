@@ -1,11 +1,15 @@
 // ensemble.hip — batched ensembles of independent realisations of uneven sizes.
 //
-// The reference runs one realisation per process (main.cu:84-252).  Here an ensemble owns one
-// engine per member; one RK3 step of the whole ensemble is captured once into a hipGraph whose
-// branches are the members' kernel sequences (fork/join with events on a capture stream), so the
-// small members run concurrently and fill the device, and each replay costs one graph launch.
-// Every member uses the tuned pass variants of its size (tuning table loaded at creation).
+// The reference runs one realisation per process (main.cu:84-252).  Here the members of one grid
+// (same sizes, scalars, precision, flags, box and dt; each with its own Re, Sc, theta and seed)
+// share one batched engine: their state arrays lie back to back and every pass of the step is one
+// launch over all of them, so 32 members of 64^3 cost the launches of one.  One RK3 step of the
+// whole ensemble is captured once into a hipGraph whose branches are the engines' kernel sequences
+// (fork/join with events on a capture stream), and each replay costs one graph launch.  Every
+// engine uses the tuned pass variants of its grid (tuning table loaded at creation).
+// GNSS_ENSEMBLE_BATCH=0 gives every member an engine of its own.
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -19,9 +23,10 @@ using gnss::EngineBase;
 using gnss::Error;
 
 struct gnss_ens {
-  std::vector<EngineBase*> m;
-  std::vector<gnss_params> p;
-  std::vector<hipEvent_t> done;
+  std::vector<EngineBase*> m;        // engines (one per group of members)
+  std::vector<int> eng_of, mem_of;   // member i: its engine and its index within that engine
+  std::vector<gnss_params> p;        // member parameters
+  std::vector<hipEvent_t> done;      // per engine
   hipStream_t cap = nullptr;
   hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
@@ -74,7 +79,39 @@ GNSS_API int gnss_ensemble_create(int n, const gnss_params* members, gnss_ens** 
           p.n_scalars > GNSS_MAX_SCALARS)
         throw Error(GNSS_ERR_ARG, "member " + std::to_string(i) + ": bad parameters");
       e->p.push_back(p);
-      e->m.push_back(p.precision == GNSS_FP64 ? gnss::make_engine_f64(p) : gnss::make_engine_f32(p));
+    }
+    // groups of members that one batched engine can hold, in order of first appearance
+    const char* env = getenv("GNSS_ENSEMBLE_BATCH");
+    const bool batch = !(env && env[0] == '0');
+    const unsigned solo = GNSS_FLAG_COLLOID | GNSS_FLAG_MFMA_R16 | GNSS_FLAG_UNFUSED;
+    auto same_grid = [&](const gnss_params& a, const gnss_params& b) {
+      return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.n_scalars == b.n_scalars &&
+             a.precision == b.precision && a.flags == b.flags && a.lx == b.lx && a.ly == b.ly &&
+             a.lz == b.lz && a.dt == b.dt && !(a.flags & solo);
+    };
+    std::vector<std::vector<int>> groups;
+    for (int i = 0; i < n; ++i) {
+      size_t g = groups.size();
+      if (batch)
+        for (size_t k = 0; k < groups.size(); ++k)
+          if (same_grid(e->p[groups[k][0]], e->p[i])) g = k;
+      if (g == groups.size()) groups.emplace_back();
+      groups[g].push_back(i);
+    }
+    e->eng_of.assign(n, 0);
+    e->mem_of.assign(n, 0);
+    for (const std::vector<int>& grp : groups) {
+      std::vector<gnss_params> gp;
+      for (int i : grp) gp.push_back(e->p[i]);
+      const bool f64 = gp[0].precision == GNSS_FP64;
+      EngineBase* x = gp.size() == 1 ? (f64 ? gnss::make_engine_f64(gp[0]) : gnss::make_engine_f32(gp[0]))
+                                     : (f64 ? gnss::make_engine_batch_f64(gp.data(), (int)gp.size())
+                                            : gnss::make_engine_batch_f32(gp.data(), (int)gp.size()));
+      e->m.push_back(x);
+      for (size_t k = 0; k < grp.size(); ++k) {
+        e->eng_of[grp[k]] = (int)e->m.size() - 1;
+        e->mem_of[grp[k]] = (int)k;
+      }
       hipEvent_t ev;
       GNSS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
       e->done.push_back(ev);
@@ -154,14 +191,18 @@ GNSS_API int gnss_ensemble_step(gnss_ens* e, int nsteps) {
 
 GNSS_API int gnss_ensemble_stats(gnss_ens* e, int i, gnss_stats_out* o, double* yprof, double* spectrum) {
   return eguard(e, [&] {
-    if (i < 0 || i >= (int)e->m.size()) throw Error(GNSS_ERR_ARG, "member out of range");
-    e->m[i]->stats(o, yprof, spectrum);
+    if (i < 0 || i >= (int)e->p.size()) throw Error(GNSS_ERR_ARG, "member out of range");
+    EngineBase* x = e->m[e->eng_of[i]];
+    x->select_member(e->mem_of[i]);
+    x->stats(o, yprof, spectrum);
   });
 }
 
 GNSS_API int gnss_ensemble_get_field(gnss_ens* e, int i, int f, int space, double* host) {
   return eguard(e, [&] {
-    if (i < 0 || i >= (int)e->m.size() || !host) throw Error(GNSS_ERR_ARG, "member out of range");
-    e->m[i]->get_field(f, space, host);
+    if (i < 0 || i >= (int)e->p.size() || !host) throw Error(GNSS_ERR_ARG, "member out of range");
+    EngineBase* x = e->m[e->eng_of[i]];
+    x->select_member(e->mem_of[i]);
+    x->get_field(f, space, host);
   });
 }

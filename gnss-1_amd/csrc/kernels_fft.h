@@ -31,6 +31,15 @@ __device__ __forceinline__ int outer_index(int o, int keep, int n) {
   if (keep < 0) return o;
   return o <= keep ? o : n - (2 * keep + 1) + o;
 }
+// Batched contexts hold several members of one grid back to back (mstride elements apart) and a
+// pass runs over every member's outer indices (nouter = members x nouter_m).  Splits outer index o
+// into the member offset (returned) and the index within the member (left in o).
+__device__ __forceinline__ long long member_split(int& o, int nouter_m, long long mstride) {
+  if (nouter_m <= 0) return 0;
+  const int m = o / nouter_m;
+  o -= m * nouter_m;
+  return m * mstride;
+}
 __device__ __forceinline__ bool row_kept(int p, int n, int keep) {
   if (keep < 0) return true;
   const int m = mode_of(p, n);
@@ -47,6 +56,8 @@ struct ColArgs {
   int outer_n;
   int in_keep;         // -1 all rows, else rows with |mode| <= in_keep are loaded (others are 0)
   int out_keep;        // -1 all rows, else only rows with |mode| <= out_keep are stored
+  int nouter_m = 0;    // batched contexts: outer indices per member (0: one member)
+  long long mstride = 0;  // batched contexts: elements between members
 };
 
 // C2C pass along a strided axis of length L, in place.  Block = G x TPC x C threads.
@@ -67,8 +78,10 @@ k_col(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw, ColArgs a) {
   const int chunk = blockIdx.x % a.nchunks;
   const int o = (blockIdx.x / a.nchunks) * a.G + g;
   const bool active = o < a.nouter;
-  const long long io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
-  cx<T>* base = data + io * a.ostride + (long long)chunk * C + c;
+  int om = active ? o : 0;
+  const long long mo = member_split(om, a.nouter_m, a.mstride);
+  const long long io = outer_index(om, a.outer_keep, a.outer_n);
+  cx<T>* base = data + mo + io * a.ostride + (long long)chunk * C + c;
   cx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -110,8 +123,10 @@ k_col_multi(FieldList<T> fl, const cx<T>* __restrict__ tw, ColArgs a, int tpf, i
     const int chunk = lt % a.nchunks;
     const int o = (lt / a.nchunks) * a.G + g;
     active = o < a.nouter;
-    const long long io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
-    return fl.p[f] + io * a.ostride + (long long)chunk * C + c;
+    int om = active ? o : 0;
+    const long long mo = member_split(om, a.nouter_m, a.mstride);
+    const long long io = outer_index(om, a.outer_keep, a.outer_n);
+    return fl.p[f] + mo + io * a.ostride + (long long)chunk * C + c;
   };
   auto load = [&](int tile, cx<T>* v) {
     bool active;

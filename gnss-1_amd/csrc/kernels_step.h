@@ -111,6 +111,10 @@ struct RkCoef {
   double a, b, tau, dt;   // dq <- a dq + dt rhs;  q <- E (q + b dq);  dq <- E dq;  E = exp(-nu k^2 tau)
   int first, last;        // first: a = 0 (dq not read); last: dq not written
   double nu[MAXF];        // viscosity (velocity) and diffusivities (scalars)
+  // batched contexts (k_rk): members nbox elements of the box apart in the launch, mstride
+  // elements apart in memory, their viscosities/diffusivities in nu_m[member * MAXF + field]
+  long long nbox, mstride;
+  const double* nu_m;
 };
 
 // Projection + scalar sign (computeRHSKernel_mgpu, solver.cu:461-521), mask (solver.cu:17-42)
@@ -119,11 +123,16 @@ template <typename T>
 __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kxv,
                      const T* __restrict__ kyv, const T* __restrict__ kzv, RkCoef c, int F,
                      long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  int m = 0;
+  if (c.nbox > 0) {
+    m = (int)(i / c.nbox);
+    i -= m * c.nbox;
+  }
   int ix, iy, iz;
   box_index(g, b, i, ix, iy, iz);
-  const long long o = ((long long)ix * g.ny + iy) * g.kzs + iz;
+  const long long o = m * c.mstride + ((long long)ix * g.ny + iy) * g.kzs + iz;
   const int mz = mode_of(iz, g.nz);
   if (iz >= g.nzh || !kept(g, mode_of(ix, g.nx), mode_of(iy, g.ny), mz)) {
     for (int f = 0; f < F; ++f) {
@@ -156,7 +165,7 @@ __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const 
 #pragma unroll
   for (int f = 0; f < MAXF; ++f) {
     if (f >= F) break;
-    const T e = dexp<T>(-(T)c.nu[f] * k2 * (T)c.tau);
+    const T e = dexp<T>(-(T)(c.nu_m ? c.nu_m[m * MAXF + f] : c.nu[f]) * k2 * (T)c.tau);
     cx<T> d = cscale(rhs[f], dt);
     if (!c.first) {
       const cx<T> old = ld_s(&dq.p[f][o]);
@@ -654,8 +663,10 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     const int chunk = lt % a.nchunks;
     const int o = (lt / a.nchunks) * a.G + g;
     active = o < a.nouter;
-    const int io = outer_index(active ? o : 0, a.outer_keep, a.outer_n);
-    return (unsigned)io * (unsigned)a.ostride + (unsigned)(chunk * C + c);
+    int om = active ? o : 0;
+    const unsigned mo = (unsigned)member_split(om, a.nouter_m, a.mstride);
+    const int io = outer_index(om, a.outer_keep, a.outer_n);
+    return mo + (unsigned)io * (unsigned)a.ostride + (unsigned)(chunk * C + c);
   };
   auto load_col = [&](const cx<T>* __restrict__ src, int tile, cx<T>* va) {
     bool active;

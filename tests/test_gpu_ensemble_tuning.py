@@ -32,6 +32,29 @@ def test_ensemble_members_match_single_runs(g, engine, oracle):
         assert rel(ens.get_spectral(5, f), o.get_spectral(f)) < 1e-12
 
 
+def test_batched_ensemble_equals_unbatched(g, engine, monkeypatch):
+    """Members of one grid share one batched engine (their states back to back, one launch per
+    pass for all of them); with GNSS_ENSEMBLE_BATCH=0 every member has an engine of its own.  The
+    per-element arithmetic is the same (late-derivative x mode, per-member viscosity), so the fields
+    and statistics agree bit for bit."""
+    cfgs = g.ensemble_configs(counts=((64, 5), (96, 3), (128, 2)))
+    fields, stats = [], []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GNSS_ENSEMBLE_BATCH", flag)
+        print(f"\n[ensemble GNSS_ENSEMBLE_BATCH={flag}, {len(cfgs)} members]", flush=True)
+        ens = g.Ensemble(cfgs, engine)
+        ens.init()
+        ens.step(3)
+        fields.append([[ens.get_spectral(i, f) for f in range(4)] for i in range(len(cfgs))])
+        stats.append([ens.stats(i) for i in range(len(cfgs))])
+        ens.close()
+    for i in range(len(cfgs)):
+        for f in range(4):
+            assert np.array_equal(fields[0][i][f], fields[1][i][f]), (i, f)
+        for k in ("ke", "epsilon", "cfl", "divergence", "t", "step"):
+            assert stats[0][i][k] == stats[1][i][k], (i, k)
+
+
 def test_ensemble_fp32_uneven(g, engine):
     cfgs = g.ensemble_configs(counts=((64, 4), (96, 2), (128, 1)))
     ens = g.Ensemble(cfgs, engine)
