@@ -216,8 +216,10 @@ int gnss_autotune(gnss_ctx* ctx, int reps, char* report, int len);
 int gnss_set_tuning(gnss_ctx* ctx, const char* spec);
 
 /* Batched ensemble of independent realisations of uneven sizes (no reference counterpart:
- *   the reference runs one realisation per process).  Members share one stream; one RK3 step
- *   of the whole ensemble is one hipGraph launch. */
+ *   the reference runs one realisation per process).  Members of one grid (same sizes, scalars,
+ *   precision, flags, box and dt) share one batched engine whose passes cover all of them in one
+ *   launch (GNSS_ENSEMBLE_BATCH=0 in the environment: one engine per member); one RK3 step of the
+ *   whole ensemble is one hipGraph launch.  Member indices follow the order given here. */
 typedef struct gnss_ens gnss_ens;
 int gnss_ensemble_create(int n, const gnss_params* members, gnss_ens** out);
 int gnss_ensemble_destroy(gnss_ens* ens);
