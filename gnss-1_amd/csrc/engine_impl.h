@@ -745,20 +745,27 @@ class Engine : public EngineBase {
     std::string path;
     if (const char* e = getenv("GNSS_TUNING")) path = e;
     else path = library_dir() + "/../tuning/gnss_tuning.json";
-    FILE* f = fopen(path.c_str(), "r");
-    if (!f) return;
-    const std::string key = "\"" + tuning_key() + "\"";
-    char line[512];
-    while (fgets(line, sizeof line, f)) {
-      if (strstr(line, key.c_str())) {
-        try {
-          set_tuning(strchr(line, '{') ? strchr(line, '{') : line);
-        } catch (const Error&) {
+    // a batched context first looks for its own entry ("<key>b": its launches cover every member,
+    // so other tiles can win), then for the grid's
+    for (int pass = (nb_ > 1 ? 0 : 1); pass < 2; ++pass) {
+      FILE* f = fopen(path.c_str(), "r");
+      if (!f) return;
+      const std::string key = "\"" + tuning_key() + (pass == 0 ? "b" : "") + "\"";
+      char line[512];
+      bool found = false;
+      while (fgets(line, sizeof line, f)) {
+        if (strstr(line, key.c_str())) {
+          found = true;
+          try {
+            set_tuning(strchr(line, '{') ? strchr(line, '{') : line);
+          } catch (const Error&) {
+          }
+          break;
         }
-        break;
       }
+      fclose(f);
+      if (found) return;
     }
-    fclose(f);
   }
 
   // -------------------------------------------------------------------------------------------

@@ -12,6 +12,7 @@ def test_table_is_valid():
     assert t, "empty tuning table"
     for key, v in t.items():
         prec, nx, ny, nz, s = key.split()
+        s = s[:-1] if s.endswith("b") else s  # "b": entry of a batched (ensemble) context of the grid
         assert prec in ("f32", "f64") and int(s.rstrip("c")) >= (2 if s.endswith("c") else 0)
         assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp", "kp", "cb", "lp"}
         assert v.get("cb", 0) in (0, 1) and v.get("lp", 1) in (0, 1, 2)
