@@ -477,13 +477,13 @@ class Engine : public EngineBase {
     return b;
   }
   std::string tuning_string() const {
-    char b[96];
-    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d cb=%d lp=%d", cx_, cy_, zf_, xf_, cp_, xc_,
-             zp_, kp_, cb_, lp_);
+    char b[128];
+    snprintf(b, sizeof b, "cx=%d cy=%d zf=%d xf=%d cp=%d xc=%d zp=%d kp=%d cb=%d lp=%d yc=%d", cx_, cy_, zf_, xf_, cp_,
+             xc_, zp_, kp_, cb_, lp_, yc_);
     return b;
   }
   void set_tuning(const std::string& spec) override {
-    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_, cb = cb_, lp = lp_;
+    int cx = cx_, cy = cy_, zf = zf_, xf = xf_, cp = cp_, xc = xc_, zp = zp_, kp = kp_, cb = cb_, lp = lp_, yc = yc_;
     const char* p = spec.c_str();
     const char* q;
     if ((q = strstr(p, "cp=")) || (q = strstr(p, "\"cp\": "))) cp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
@@ -496,9 +496,10 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "kp=")) || (q = strstr(p, "\"kp\": "))) kp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "cb=")) || (q = strstr(p, "\"cb\": "))) cb = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if ((q = strstr(p, "lp=")) || (q = strstr(p, "\"lp\": "))) lp = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
+    if ((q = strstr(p, "yc=")) || (q = strstr(p, "\"yc\": "))) yc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > max_xmode() ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1 || lp < 0 ||
-        lp > 2)
+        lp > 2 || (yc != 0 && !ychunk_ok(yc)))
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     cx_ = cx;
     cy_ = cy;
@@ -512,6 +513,7 @@ class Engine : public EngineBase {
     kp_ = kp;
     cb_ = cb;
     lp_ = lp;
+    yc_ = yc;
   }
 
   // -------------------------------------------------------------------------------------------
@@ -683,6 +685,7 @@ class Engine : public EngineBase {
         }
         // x-pass mode and tile, jointly with one launch per pass (cb = 1) or per field (cb = 0)
         int best = xf_, best_xc = xc_, best_cb = cb_, best_lp = lp_;
+        yc_ = 0;  // the x-mode search times the late y passes with the x tile
         double best_ms = 1e30;
         for (int b = 0; b <= 1; ++b) {
           for (int v = 0; v <= max_xmode(); ++v) {
@@ -707,6 +710,22 @@ class Engine : public EngineBase {
         xc_ = best_xc;
         cb_ = best_cb;
         lp_ = best_lp;
+        // late modes: the y inverse passes' own tile
+        yc_ = 0;
+        if (best >= 4) {
+          for (int c : {4, 8, 16}) {
+            if (c == xc_ || !ychunk_ok(c)) continue;
+            yc_ = c;
+            const double ms = time_it([&] { substage(1); });
+            snprintf(buf, sizeof buf, ", \"yc=%d\": %.5f", c, ms);
+            rep += buf;
+            if (ms < best_ms) {
+              best_ms = ms;
+            } else {
+              yc_ = 0;
+            }
+          }
+        }
         for (int f = 0; f < F_; ++f) {
           GNSS_HIP(hipMemcpyAsync(q_[f], snap[2 * f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
           GNSS_HIP(hipMemcpyAsync(dq_[f], snap[2 * f + 1], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
@@ -733,6 +752,10 @@ class Engine : public EngineBase {
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
   bool xchunk_ok(int c) const {
     return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(nx_, c);
+  }
+  // column tile of the late y inverse passes (0: the x tile xc)
+  bool ychunk_ok(int c) const {
+    return (sizeof(T) == 8 ? (c == 4 || c == 8) : (c == 8 || c == 16)) && chunk_ok(ny_, c);
   }
   bool chunk_ok(int L, int c) const {
     if (sizeof(T) == 8 ? (c != 4 && c != 8) : (c != 8 && c != 16)) return false;
@@ -1320,7 +1343,7 @@ class Engine : public EngineBase {
       put(ly, nye, ws_[8 + 3 * q], nullptr, ws_[8 + 3 * q], ws_[7 + 3 * q], copy_d);
       put(ly, nye, ws_[6 + 3 * q], nullptr, ws_[6 + 3 * q], nullptr, copy);
     }
-    const int CC = pair_chunk();
+    const int CC = pair_chunk(), CY = yc_ ? yc_ : CC;
     ColArgs ax;
     ax.estride = (long long)ny_ * kzs_;
     ax.ostride = kzs_;
@@ -1337,14 +1360,14 @@ class Engine : public EngineBase {
     ay.outer_keep = -1;
     ay.outer_n = nx_;
     ay.nouter = nx_;
-    ay.nchunks = kz_lim_ / CC;
+    ay.nchunks = kz_lim_ / CY;
     ay.in_keep = g_.keep_y;
     ay.out_keep = -1;
     ay.G = 1;
     batch_outer(ax);
     batch_outer(ay);
-    pair_pass(lx, nxe, ax, nx_, tkx_, "fft_x_inv_late");
-    pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late");
+    pair_pass(lx, nxe, ax, nx_, tkx_, "fft_x_inv_late", CC);
+    pair_pass(ly, nye, ay, ny_, tky_, "fft_y_inv_late", CY);
   }
   int pair_chunk() const { return xc_; }
   // A pass of a batched step covers the outer indices of every member.
@@ -1359,8 +1382,8 @@ class Engine : public EngineBase {
   // single-output entries that read the inputs again, the second outputs first (they read fields
   // the first outputs overwrite in place).  Then one launch per (inputs, outputs) kind with cb,
   // else one per entry.
-  void pair_pass(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name) {
-    if (lp_) return pair_kinds(li, n, a, L, kv, name);
+  void pair_pass(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name, int CC) {
+    if (lp_) return pair_kinds(li, n, a, L, kv, name, CC);
     PairList<T> first{}, second{};
     int n1 = 0, n2 = 0;
     for (int i = 0; i < n; ++i) {
@@ -1380,11 +1403,10 @@ class Engine : public EngineBase {
         second.op[n2++] = o1;
       }
     }
-    pair_kinds(second, n2, a, L, kv, name);
-    pair_kinds(first, n1, a, L, kv, name);
+    pair_kinds(second, n2, a, L, kv, name, CC);
+    pair_kinds(first, n1, a, L, kv, name, CC);
   }
-  void pair_kinds(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name) {
-    const int CC = pair_chunk();
+  void pair_kinds(const PairList<T>& li, int n, ColArgs a, int L, const T* kv, const char* name, int CC) {
     auto rows = [&](int keep) { return keep < 0 ? L : std::min(L, 2 * keep + 1); };
     const double col = (double)a.nouter * a.nchunks * CC * sizeof(C);
     static const int KIN[4] = {1, 1, 2, 2}, KOUT[4] = {1, 2, 2, 1};
@@ -1678,7 +1700,7 @@ class Engine : public EngineBase {
   int cp_ = 0, xc_ = 8, zp_ = 0, kp_ = 0, cb_ = 0, lp_ = 1, n_cu_ = 256;
   // batched contexts: members held (nb_), members the launches in flight cover (bm_: nb_ inside a
   // step, 1 elsewhere), member parameters, base pointers of the state, per-member viscosities
-  int nb_ = 1, bm_ = 1;
+  int nb_ = 1, bm_ = 1, yc_ = 0;
   std::vector<gnss_params> mp_;
   std::vector<C*> qb_, dqb_;
   double* d_num_ = nullptr;

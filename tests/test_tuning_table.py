@@ -14,11 +14,12 @@ def test_table_is_valid():
         prec, nx, ny, nz, s = key.split()
         s = s[:-1] if s.endswith("b") else s  # "b": entry of a batched (ensemble) context of the grid
         assert prec in ("f32", "f64") and int(s.rstrip("c")) >= (2 if s.endswith("c") else 0)
-        assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp", "kp", "cb", "lp"}
+        assert {"cx", "cy", "zf", "xf", "cp"} <= set(v) <= {"cx", "cy", "zf", "xf", "cp", "xc", "zp", "kp", "cb", "lp", "yc"}
         assert v.get("cb", 0) in (0, 1) and v.get("lp", 1) in (0, 1, 2)
         assert v.get("kp", 0) in range(5)
         allowed = {4, 8} if prec == "f64" else {8, 16}
         assert v["cx"] in allowed and v["cy"] in allowed and v.get("xc", v["cx"]) in allowed
+        assert v.get("yc", 0) in allowed | {0}
         # x modes 4 and 5 (late derivatives) are not built for the colloid
         assert v["xf"] in ((0, 1, 2, 3) if s.endswith("c") else (0, 1, 2, 3, 4, 5))
         assert v["cp"] in (0, 1) and v.get("zp", 0) in (0, 1)
