@@ -12,6 +12,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -89,12 +91,20 @@ GNSS_API int gnss_ensemble_create(int n, const gnss_params* members, gnss_ens** 
              a.precision == b.precision && a.flags == b.flags && a.lx == b.lx && a.ly == b.ly &&
              a.lz == b.lz && a.dt == b.dt && !(a.flags & solo);
     };
+    // Only grids up to 256^3 are batched (larger members fill the device by themselves), and a
+    // batched engine holds fewer than 2^31 elements per field (the late passes' 32-bit offsets;
+    // the row padding of a field is at most 5 tiles of 16): larger groups are split.
+    auto max_members = [](const gnss_params& a) -> size_t {
+      if ((double)a.nx * a.ny * a.nz > 256.0 * 256.0 * 256.0) return 1;
+      const double field = (double)a.nx * a.ny * (a.nz / 2 + 1 + 6 * 16);
+      return (size_t)std::max(1.0, std::floor(2147483647.0 / field));
+    };
     std::vector<std::vector<int>> groups;
     for (int i = 0; i < n; ++i) {
       size_t g = groups.size();
       if (batch)
         for (size_t k = 0; k < groups.size(); ++k)
-          if (same_grid(e->p[groups[k][0]], e->p[i])) g = k;
+          if (same_grid(e->p[groups[k][0]], e->p[i]) && groups[k].size() < max_members(e->p[i])) g = k;
       if (g == groups.size()) groups.emplace_back();
       groups[g].push_back(i);
     }
