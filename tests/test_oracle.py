@@ -1,8 +1,9 @@
 """Pins the FP64 oracle (oracle/gnss_oracle.c) before it is trusted as the engine's checker.
 
-The reference ships no tests or vectors (SURVEY: no test directory, cuFFT unpinned), so every
-check here is a brute-force computation, a committed library-generated vector, a closed form or
-an invariant — chosen so that a dropped term, a wrong sign or index, or a transposed operand fails:
+The reference ships no tests or vectors (it has no test directory, and nothing it holds pins its
+FFT library, cuFFT), so every check here is a brute-force computation, a committed
+library-generated vector, a closed form or an invariant — chosen so that a dropped term, a wrong
+sign or index, or a transposed operand fails:
 
   FFT            brute-force DFT (explicit exponentials) on tiny grids; numpy.fft.rfftn vectors
                  committed in tests/golden/fft_golden.npz; round trip; Parseval
