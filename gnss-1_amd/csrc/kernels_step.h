@@ -642,8 +642,8 @@ template <typename T> struct PairList {
 // the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch.  Two outputs go
 // through the radix stages together, in two LDS regions (SEQ = false), or one after the other
 // through one region (SEQ = true: half the LDS, and the next tile's A is loaded once the last
-// output is formed, before it transforms).  Element offsets are 32-bit: a field holds fewer than
-// 2^31 elements (checked by the host).  An entry may write its outputs over its own inputs (each
+// output is formed, before it transforms).  Element offsets are 32-bit: a field (all members of a
+// batched context together) holds fewer than 2^31 elements (checked by the host).  An entry may write its outputs over its own inputs (each
 // tile is read before it is written, by the one block that owns it, and both outputs are formed
 // from registers); no entry may read a field another entry writes.
 template <typename T, int L, int C, int NIN, int NOUT, bool SEQ>
