@@ -100,7 +100,7 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(cfg, threads, steps=2):
+def cpu_baseline(cfg, threads, steps=3):
     """Bounded sample: `steps` FP64 RK3 steps of the same workload on the oracle (FFTW3 API)."""
     lib = gnss.load(ORACLE_LIB)
     lib.gnss_oracle_set_threads.argtypes = [gnss.ctypes.c_int]

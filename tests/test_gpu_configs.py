@@ -6,6 +6,7 @@ steps run engine-only in gnss-1_amd/run_configs.py).  The oracle runs on the hos
 With GNSS_FULL_CONFIGS=1 the whole configuration (200 steps, statistics every 20) is compared
 against the oracle as well; it takes minutes of host time, so it is opt-in."""
 import os
+import time
 
 import numpy as np
 import pytest
@@ -52,12 +53,16 @@ def test_jet256_full_configuration_vs_oracle(g, engine, oracle):
     oracle.gnss_oracle_set_backend.argtypes = [g.ctypes.c_int]
     oracle.gnss_oracle_set_backend(1)
     worst = {"stats": 0.0, "profiles": 0.0, "spectrum": 0.0}
+    ta = tb = 0.0
     try:
         a, b = g.Solver(cfg, engine).setup(), g.Solver(cfg, oracle).setup()
         for chunk in range(11):
             if chunk:
+                t0 = time.perf_counter()
                 a.step(20)
+                t1 = time.perf_counter()
                 b.step(20)
+                ta, tb = ta + t1 - t0, tb + time.perf_counter() - t1
             sa, ya, pa = a.stats()
             sb, yb, pb = b.stats()
             ds = max(abs(sa[k] - sb[k]) / abs(sb[k]) for k in ["ke", "vrms", "epsilon", "enstrophy", "lint", "cfl", "umax"])
@@ -73,6 +78,38 @@ def test_jet256_full_configuration_vs_oracle(g, engine, oracle):
         print(f"  fields after 200 steps, rel-L2 (u, v, w, phi0, phi1): {', '.join('%.1e' % x for x in fr)}; "
               f"worst stats {worst['stats']:.1e}, profiles {worst['profiles']:.1e}, spectrum {worst['spectrum']:.1e}",
               flush=True)
+        assert max(fr) <= 1e-12, fr
+        print(f"  200 steps (statistics excluded): engine {200 / ta:.1f} steps/s, oracle {200 / tb:.3f} steps/s "
+              f"({_oracle_desc(g, oracle)})", flush=True)
+    finally:
+        oracle.gnss_oracle_set_backend(0)
+
+
+def _oracle_desc(g, oracle):
+    oracle.gnss_oracle_backend_name.restype = g.ctypes.c_char_p
+    oracle.gnss_oracle_max_threads.restype = g.ctypes.c_int
+    return f"FFT {oracle.gnss_oracle_backend_name().decode()}, {oracle.gnss_oracle_max_threads()} threads"
+
+
+@pytest.mark.skipif(os.environ.get("GNSS_FULL_CONFIGS") != "1",
+                    reason="times the oracle over the whole 64^3 configuration; set GNSS_FULL_CONFIGS=1")
+def test_jet64_full_configuration_timed(g, engine, oracle):
+    """The whole 64^3 configuration (100 FP64 steps), engine and oracle (threaded FFTW3 API) timed
+    over the step loop only, and their fields compared (rel-L2 <= 1e-12)."""
+    print("\n[jet64 fp64, 100 steps, engine and oracle timed]", flush=True)
+    cfg = g.config("jet64")
+    oracle.gnss_oracle_set_backend.argtypes = [g.ctypes.c_int]
+    oracle.gnss_oracle_set_backend(1)
+    try:
+        a, b = g.Solver(cfg, engine).setup(), g.Solver(cfg, oracle).setup()
+        t0 = time.perf_counter()
+        a.step(100)
+        t1 = time.perf_counter()
+        b.step(100)
+        t2 = time.perf_counter()
+        fr = [rel(a.get_spectral(f), b.get_spectral(f)) for f in range(4)]
+        print(f"  engine {100 / (t1 - t0):.1f} steps/s, oracle {100 / (t2 - t1):.2f} steps/s "
+              f"({_oracle_desc(g, oracle)}); fields rel-L2 {', '.join('%.1e' % x for x in fr)}", flush=True)
         assert max(fr) <= 1e-12, fr
     finally:
         oracle.gnss_oracle_set_backend(0)
