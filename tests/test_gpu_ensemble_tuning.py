@@ -74,6 +74,31 @@ def test_batched_ensemble_equals_unbatched(g, engine, monkeypatch, tmp_path):
             assert rel(fs[i][f], fu[i][f]) <= 2e-6, (i, f)
 
 
+def test_ensemble_mixed_members_grouping(g, engine):
+    """An ensemble mixing precisions, grids, scalar counts and a colloid member: members of one grid
+    are batched, the colloid member stays on an engine of its own, and every member equals its
+    stand-alone run (FP64 1e-13, FP32 2e-6: batched engines may take other pass variants)."""
+    f32 = g.ensemble_configs(counts=((64, 2),))
+    f64 = g.ensemble_configs(counts=((32, 2),), precision=g.FP64, seed=7)
+    two = [g.Config(**{**c.__dict__, "sc": (0.7, 2.0), "seed": c.seed + 100}) for c in f32]
+    col = [g.with_colloid(f32[0], alpha=0.02, theta_c=0.08)]
+    cfgs = [f32[0], f64[0], two[0], col[0], f32[1], f64[1], two[1]]
+    print(f"\n[ensemble of {len(cfgs)} mixed members]", flush=True)
+    ens = g.Ensemble(cfgs, engine)
+    ens.init()
+    ens.step(3)
+    for i, c in enumerate(cfgs):
+        s = g.Solver(c, engine).setup()
+        s.step(3)
+        tol = 1e-13 if c.precision == g.FP64 else 2e-6
+        for f in range(3 + len(c.sc)):
+            assert rel(ens.get_spectral(i, f), s.get_spectral(f)) <= tol, (i, f)
+        st = ens.stats(i)
+        assert st["step"] == 3
+        s.close()
+    ens.close()
+
+
 def test_ensemble_fp32_uneven(g, engine):
     cfgs = g.ensemble_configs(counts=((64, 4), (96, 2), (128, 1)))
     ens = g.Ensemble(cfgs, engine)
