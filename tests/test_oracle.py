@@ -12,7 +12,10 @@ sign or index, or a transposed operand fails:
   step           single-mode exact viscous decay (integrating factor), scalar diffusion, 2D
                  Taylor-Green exact decay (projection of a pure-gradient u x w), third-order
                  convergence of RK3, inviscid energy and scalar-variance conservation,
-                 divergence-free evolution, Galilean sign check of the scalar advection
+                 divergence-free evolution, Galilean sign check of the scalar advection and of
+                 the velocity advection (a translating shear wave), one-step tendency of a 3D
+                 field whose u x w is not a gradient (each vorticity component, the cross
+                 product order, u . grad phi per component); FFTW3-API backend = built-in FFT
   statistics     single-mode closed forms (KE, epsilon, chi, spectrum shell, profiles, CFL)
   initial cond.  Hermitian symmetry, divergence, band limit, noise rms, jet DFT, seed determinism
 """
@@ -200,6 +203,99 @@ def test_scalar_advection_sign(g, oracle):
     t = steps * dt
     expect = np.cos(2 * (X - U * t)) + 0.5 * np.sin(3 * (X - U * t) + Y)
     assert np.abs(s.get_physical(3) - expect).max() < 1e-9
+
+
+def test_shear_wave_galilean_translation(g, oracle):
+    """u = (U, eps sin kx, 0): omega = (0, 0, eps k cos kx) and the y component of u x omega is
+    -U eps k cos kx, so v = eps exp(-nu k^2 t) sin(k (x - U t)) travels at +U.  With omega negated
+    it would travel at -U; with omega_z formed from the wrong derivatives it would not move.  The
+    x component (eps^2 k / 2) sin 2kx is a pure gradient and the projection removes it, so u stays
+    uniform (solver.cu:68-114 vorticity, 139-164 cross product, 504-511 projection)."""
+    n, re, dt, steps, U, eps, k = (16, 8, 8), 200.0, 1e-3, 60, 0.8, 0.05, 2
+    s = solver(g, oracle, n=n, re=re, sc=(), dt=dt, noise_amp=0.0)
+    s.init_wavenumbers()
+    X, Y, Z = grid(n)
+    set_physical_state(s, [U + 0 * X, eps * np.sin(k * X), 0 * X])
+    s.step(steps)
+    t = steps * dt
+    for f in range(3):
+        s.inverse(f)
+    expect_v = eps * np.exp(-k * k * t / re) * np.sin(k * (X - U * t))
+    assert np.abs(s.get_physical(1) - expect_v).max() < 1e-12
+    assert np.abs(s.get_physical(0) - U).max() < 1e-13
+    assert np.abs(s.get_physical(2)).max() < 1e-14
+
+
+def test_velocity_and_scalar_tendency_3d(g, oracle):
+    """One small step from u = (a sin z, b sin x, c sin y), phi = cos x + 2 cos y + 3 cos z.
+
+    omega = curl u = (c cos y, a cos z, b cos x), and
+    u x omega = (b^2 sin x cos x - c a sin y cos z,  c^2 sin y cos y - a b sin z cos x,
+                 a^2 sin z cos z - b c sin x cos y).
+    The squared terms are gradients (removed by the projection); the mixed terms are
+    perpendicular to their wavevectors and survive, so with nu = 1/Re and |k| = 1
+        du/dt = -c a sin y cos z - nu a sin z,   dv/dt = -a b sin z cos x - nu b sin x,
+        dw/dt = -b c sin x cos y - nu c sin y,
+        dphi/dt = -u . grad phi + D lap phi
+                = a sin z sin x + 2 b sin x sin y + 3 c sin y sin z - D (cos x + 2 cos y + 3 cos z).
+    u x omega is not a gradient here, so this pins the sign of every vorticity component, the
+    order of the cross product and which velocity component multiplies which gradient (a sign
+    flip or a swapped component changes the tendency by O(1), against the O(dt) truncation)."""
+    n, re, sc, dt = (16, 16, 16), 10.0, 0.5, 1e-6
+    a, b, c = 0.7, -1.1, 1.3
+    nu, D = 1.0 / re, 1.0 / (re * sc)
+    s = solver(g, oracle, n=n, re=re, sc=(sc,), dt=dt, noise_amp=0.0)
+    s.init_wavenumbers()
+    X, Y, Z = grid(n)
+    q0 = [a * np.sin(Z), b * np.sin(X), c * np.sin(Y), np.cos(X) + 2 * np.cos(Y) + 3 * np.cos(Z)]
+    set_physical_state(s, q0)
+    s.step(1)
+    rate = []
+    for f in range(4):
+        s.inverse(f)
+        rate.append((s.get_physical(f) - q0[f]) / dt)
+    expect = [
+        -c * a * np.sin(Y) * np.cos(Z) - nu * a * np.sin(Z),
+        -a * b * np.sin(Z) * np.cos(X) - nu * b * np.sin(X),
+        -b * c * np.sin(X) * np.cos(Y) - nu * c * np.sin(Y),
+        a * np.sin(Z) * np.sin(X) + 2 * b * np.sin(X) * np.sin(Y) + 3 * c * np.sin(Y) * np.sin(Z)
+        - D * (np.cos(X) + 2 * np.cos(Y) + 3 * np.cos(Z)),
+    ]
+    for f in range(4):
+        scale = np.abs(expect[f]).max()
+        assert np.abs(rate[f] - expect[f]).max() < 1e-4 * scale, f
+
+
+def test_fftw_backend_matches_builtin(g, oracle):
+    """The FFTW3-API backend (used as the checker of the long 256^3 comparison) gives the
+    built-in Stockham FFT's transforms and step results."""
+    oracle.gnss_oracle_set_backend.argtypes = [g.ctypes.c_int]
+    oracle.gnss_oracle_backend_name.restype = g.ctypes.c_char_p
+    if oracle.gnss_oracle_set_backend(1) != 0:
+        oracle.gnss_oracle_set_backend(0)
+        pytest.skip("no FFTW3 API library (libfftw3 or MKL's libmkl_rt) on this host")
+    try:
+        assert oracle.gnss_oracle_backend_name().decode() != "builtin-stockham"
+        n = (12, 16, 24)
+        x = np.random.default_rng(7).standard_normal(n)
+        s = solver(g, oracle, n=n, sc=())
+        s.set_physical(0, x)
+        s.forward(0)
+        X_fftw = s.get_spectral(0)
+        s.inverse(0)
+        x_fftw = s.get_physical(0)
+        cfg = dict(n=(16, 16, 16), re=300.0, sc=(0.7,), theta=0.1, noise_amp=0.1, noise_kmax=4.0, dt=2e-3)
+        a = solver(g, oracle, **cfg).setup()
+        a.step(5)
+        step_fftw = [a.get_spectral(f) for f in range(4)]
+    finally:
+        oracle.gnss_oracle_set_backend(0)
+    assert rel(X_fftw, to_abi(brute_rfftn(x))) < 1e-14
+    assert np.abs(x_fftw - x).max() < 1e-13
+    b = solver(g, oracle, **cfg).setup()
+    b.step(5)
+    for f in range(4):
+        assert rel(step_fftw[f], b.get_spectral(f)) < 1e-13, f
 
 
 def jet_solver(g, oracle, dt, **kw):
