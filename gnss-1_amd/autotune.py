@@ -3,10 +3,17 @@ writes the winning choices to tuning/gnss_tuning.json, which gnss_create reads.
 
 Tuned per (precision, grid, scalars, colloid): the column tile widths of the strided x and y
 passes, persistent or one-block-per-tile column and z passes, the fused z pass variant, and the
-x-pass mode and its tile (`gnss_autotune` inside one context).  The row padding kp keeps the value
-the table already holds (0 by default).
+x-pass mode and its tile (`gnss_autotune` inside one context).  With --ensemble the batched
+engines of the uneven ensemble (32 x 64^3 + 16 x 96^3 + 8 x 128^3, FP32) are tuned as they run,
+all members per launch (`gnss_ensemble_autotune`), giving the "<key>b" entries.  The row padding
+kp keeps the value the table already holds (0 by default).
 
-usage: python gnss-1_amd/autotune.py [--reps 5] [--out gnss-1_amd/tuning/gnss_tuning.json] [--big]
+The table is updated in place: keys this run tunes are replaced, every other entry is kept.  The
+raw candidate timings go to --report (default gpurun_out/autotune_report.json), not beside the
+table.
+
+usage: python gnss-1_amd/autotune.py [--reps 5] [--out gnss-1_amd/tuning/gnss_tuning.json]
+                                     [--big] [--ensemble | --ensemble-only]
 """
 import argparse
 import json
@@ -36,6 +43,13 @@ def cases(big: bool):
     return out
 
 
+def read_table(path):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
 def write_table(path, table):
     with open(path, "w") as f:
         f.write("{\n")
@@ -45,26 +59,59 @@ def write_table(path, table):
         f.write("}\n")
 
 
+def choice_of(rep, old):
+    """Table entry of one tuning report; kp (not tuned) keeps its old value."""
+    ch = {k: int(v) for k, v in (kv.split("=") for kv in rep["choice"].split())}
+    ch["kp"] = int(old.get("kp", ch.get("kp", 0)))
+    return ch
+
+
+def tune_single(reps, big, table, reports):
+    for cfg in cases(big):
+        t0 = time.time()
+        s = gnss.Solver(cfg).setup()
+        rep = s.autotune(reps)
+        s.close()
+        table[rep["key"]] = choice_of(rep, table.get(rep["key"], {}))
+        reports.append(rep)
+        print(f"[autotune] {rep['key']}: {rep['choice']}  ({time.time() - t0:.1f} s)", flush=True)
+
+
+def tune_ensemble(reps, table, reports):
+    t0 = time.time()
+    ens = gnss.Ensemble(gnss.ensemble_configs())
+    ens.init()
+    for rep in ens.autotune(reps):
+        table[rep["key"]] = choice_of(rep, table.get(rep["key"], {}))
+        reports.append(rep)
+        print(f"[autotune] {rep['key']} ({rep['members']} members): {rep['choice']}", flush=True)
+    ens.close()
+    print(f"[autotune] ensemble tuned in {time.time() - t0:.1f} s", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning", "gnss_tuning.json"))
+    ap.add_argument("--report", default=os.path.join("gpurun_out", "autotune_report.json"))
     ap.add_argument("--big", action="store_true", help="also tune the 1024^3 FP32 workload")
+    ap.add_argument("--ensemble", action="store_true", help="also tune the batched ensemble engines")
+    ap.add_argument("--ensemble-only", action="store_true", help="tune only the batched ensemble engines")
     args = ap.parse_args()
-    table, reports = {}, []
-    for cfg in cases(args.big):
-        t0 = time.time()
-        s = gnss.Solver(cfg).setup()
-        rep = s.autotune(args.reps)
-        s.close()
-        choice = dict(kv.split("=") for kv in rep["choice"].split())
-        table[rep["key"]] = {k: int(v) for k, v in choice.items()}
-        reports.append(rep)
-        print(f"[autotune] {rep['key']}: {rep['choice']}  ({time.time() - t0:.1f} s)", flush=True)
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    table, reports = read_table(args.out), []
+    if not args.ensemble_only:
+        tune_single(args.reps, args.big, table, reports)
+    if args.ensemble or args.ensemble_only:
+        # the batched engines read the table's single-context entries first: write them before
+        write_table(args.out, table)
+        os.environ["GNSS_TUNING"] = os.path.abspath(args.out)
+        tune_ensemble(args.reps, table, reports)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     write_table(args.out, table)
-    with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:
-        json.dump(reports, f, indent=1)
+    if args.report:
+        os.makedirs(os.path.dirname(os.path.abspath(args.report)), exist_ok=True)
+        with open(args.report, "w") as f:
+            json.dump(reports, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -125,33 +125,15 @@ template <typename Fn> static int guard(gnss_ctx* c, Fn&& fn) {
   }
 }
 
-static bool good_len(int n) { return gnss::supported_len(n); }
-
 GNSS_API int gnss_create(const gnss_params* p, gnss_ctx** out) {
   if (!p || !out) {
     g_create_err = "null argument";
     return GNSS_ERR_ARG;
   }
   *out = nullptr;
-  char msg[256];
-  if (!good_len(p->nx) || !good_len(p->ny) || !good_len(p->nz)) {
-    snprintf(msg, sizeof msg,
-             "grid %dx%dx%d: each size must be one of 16 24 32 48 64 96 128 192 256 384 512 1024", p->nx,
-             p->ny, p->nz);
-    g_create_err = msg;
-    return GNSS_ERR_ARG;
-  }
-  if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) { g_create_err = "box lengths must be > 0"; return GNSS_ERR_ARG; }
-  if (!(p->re > 0)) { g_create_err = "re must be > 0"; return GNSS_ERR_ARG; }
-  if (!(p->dt > 0)) { g_create_err = "dt must be > 0"; return GNSS_ERR_ARG; }
-  if (p->n_scalars < 0 || p->n_scalars > GNSS_MAX_SCALARS) {
-    g_create_err = "n_scalars must be in [0, 4]";
-    return GNSS_ERR_ARG;
-  }
-  for (int s = 0; s < p->n_scalars; ++s)
-    if (!(p->sc[s] > 0)) { g_create_err = "sc must be > 0"; return GNSS_ERR_ARG; }
-  if (p->precision != GNSS_FP64 && p->precision != GNSS_FP32) {
-    g_create_err = "precision must be GNSS_FP64 or GNSS_FP32";
+  const std::string bad = gnss::params_problem(*p);
+  if (!bad.empty()) {
+    g_create_err = bad;
     return GNSS_ERR_ARG;
   }
   int ndev = 0;

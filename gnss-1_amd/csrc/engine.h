@@ -106,6 +106,22 @@ inline bool supported_len(int n) {
   }
 }
 
+// The parameter checks shared by gnss_create and gnss_ensemble_create: an empty string when the
+// set is acceptable, else what is wrong with it.
+inline std::string params_problem(const gnss_params& p) {
+  if (!supported_len(p.nx) || !supported_len(p.ny) || !supported_len(p.nz))
+    return "grid " + std::to_string(p.nx) + "x" + std::to_string(p.ny) + "x" + std::to_string(p.nz) +
+           ": each size must be one of 16 24 32 48 64 96 128 192 256 384 512 1024";
+  if (!(p.lx > 0 && p.ly > 0 && p.lz > 0)) return "box lengths must be > 0";
+  if (!(p.re > 0)) return "re must be > 0";
+  if (!(p.dt > 0)) return "dt must be > 0";
+  if (p.n_scalars < 0 || p.n_scalars > GNSS_MAX_SCALARS) return "n_scalars must be in [0, 4]";
+  for (int s = 0; s < p.n_scalars; ++s)
+    if (!(p.sc[s] > 0)) return "sc must be > 0";
+  if (p.precision != GNSS_FP64 && p.precision != GNSS_FP32) return "precision must be GNSS_FP64 or GNSS_FP32";
+  return "";
+}
+
 // Directory of the loaded engine library (for its tuning table).
 std::string library_dir();
 

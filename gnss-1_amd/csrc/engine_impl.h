@@ -501,13 +501,13 @@ class Engine : public EngineBase {
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1 || lp < 0 ||
         lp > 2 || (yc != 0 && !ychunk_ok(yc)))
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
+    if (kp != kp_ && nsp_ > 0)
+      throw Error(GNSS_ERR_ARG, "kp (row padding) is fixed when the context is created: set it in the tuning table");
     cx_ = cx;
     cy_ = cy;
     zf_ = zf;
     xf_ = xf;
     cp_ = cp;
-    if (kp != kp_ && nsp_ > 0)
-      throw Error(GNSS_ERR_ARG, "kp (row padding) is fixed when the context is created: set it in the tuning table");
     xc_ = xc;
     zp_ = zp;
     kp_ = kp;
@@ -575,7 +575,7 @@ class Engine : public EngineBase {
 
   std::string autotune(int reps) override {
     need_k();
-    if (nb_ > 1) throw Error(GNSS_ERR_UNSUPPORTED, "autotune a single-member context of the grid (the table is shared)");
+    if (nb_ > 1) return autotune_batched(reps);
     if (reps <= 0) return "{\"key\": \"" + tuning_key() + "\", \"choice\": \"" + tuning_string() + "\"}";
     const bool was_on = timer_.on();
     timer_.enable(false, s_);
@@ -744,6 +744,126 @@ class Engine : public EngineBase {
     return rep + "}, \"choice\": \"" + tuning_string() + "\"}";
   }
 
+  // A batched context's own entry ("<key>b"): its launches cover every member, so other pass
+  // variants can win than for one member of the grid.  Batched steps always run the late x mode;
+  // the fused z variant is timed alone, then the inverse passes' tile jointly with persistent or
+  // per-tile column passes (cp), one launch per pass or per field (cb) and the late passes' output
+  // form (lp), then the forward passes' tiles (cx, cy) and the late y passes' own tile (yc) — all
+  // over whole middle substages of every member, on a snapshot of the state restored afterwards.
+  std::string autotune_batched(int reps) {
+    const std::string key = tuning_key() + "b";
+    if (reps <= 0) return "{\"key\": \"" + key + "\", \"choice\": \"" + tuning_string() + "\"}";
+    const bool was_on = timer_.on();
+    timer_.enable(false, s_);
+    std::vector<void*> snap;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string rep = "{\"key\": \"" + key + "\", \"members\": " + std::to_string(nb_) + ", \"candidates\": {";
+    char buf[160];
+    select_member(0);
+    const size_t fbytes = sizeof(C) * (size_t)nsp_ * nb_;
+    try {
+      GNSS_HIP(hipEventCreate(&e0));
+      GNSS_HIP(hipEventCreate(&e1));
+      for (int f = 0; f < F_; ++f)
+        for (C* src : {qb_[f], dqb_[f]}) {
+          void* v = nullptr;
+          GNSS_HIP(hipMalloc(&v, fbytes));
+          snap.push_back(v);
+          GNSS_HIP(hipMemcpyAsync(v, src, fbytes, hipMemcpyDeviceToDevice, s_));
+        }
+      bm_ = nb_;
+      auto time_it = [&](auto&& fn) {
+        fn();  // warm-up
+        GNSS_HIP(hipEventRecord(e0, s_));
+        for (int r = 0; r < reps; ++r) fn();
+        GNSS_HIP(hipEventRecord(e1, s_));
+        GNSS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        GNSS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / reps;
+      };
+      bool first = true;
+      auto note = [&](const char* what, double ms) {
+        snprintf(buf, sizeof buf, "%s\"%s\": %.5f", first ? "" : ", ", what, ms);
+        rep += buf;
+        first = false;
+      };
+      {
+        int best = zf_;
+        double best_ms = 1e30;
+        for (int v = 0; v <= 8; ++v) {
+          if (!zvariant_ok(v)) continue;
+          zf_ = v;
+          const double ms = time_it([&] { z_fused(); });
+          snprintf(buf, sizeof buf, "zf=%d", v);
+          note(buf, ms);
+          if (ms < best_ms) { best_ms = ms; best = v; }
+        }
+        zf_ = best;
+      }
+      double best_ms = 1e30;
+      {
+        int bxc = xc_, bcp = cp_, bcb = cb_, blp = lp_;
+        yc_ = 0;
+        for (int c : {4, 8, 16}) {
+          if (!xchunk_ok(c)) continue;
+          for (int cp = 0; cp <= 1; ++cp)
+            for (int cb = 0; cb <= 1; ++cb)
+              for (int l = 0; l <= 2; ++l) {
+                xc_ = c; cp_ = cp; cb_ = cb; lp_ = l;
+                const double ms = time_it([&] { substage(1); });
+                snprintf(buf, sizeof buf, "xc=%d cp=%d cb=%d lp=%d", c, cp, cb, l);
+                note(buf, ms);
+                if (ms < best_ms) { best_ms = ms; bxc = c; bcp = cp; bcb = cb; blp = l; }
+              }
+        }
+        xc_ = bxc; cp_ = bcp; cb_ = bcb; lp_ = blp;
+      }
+      for (int axis = 0; axis < 2; ++axis) {
+        int& knob = axis == 0 ? cx_ : cy_;
+        const int keep = knob;
+        int best = knob;
+        for (int c : {4, 8, 16}) {
+          if (c == keep || !chunk_ok(axis == 0 ? nx_ : ny_, c)) continue;
+          knob = c;
+          const double ms = time_it([&] { substage(1); });
+          snprintf(buf, sizeof buf, "c%c=%d", axis == 0 ? 'x' : 'y', c);
+          note(buf, ms);
+          if (ms < best_ms) { best_ms = ms; best = c; }
+        }
+        knob = best;
+      }
+      for (int c : {4, 8, 16}) {
+        if (c == xc_ || !ychunk_ok(c)) continue;
+        const int keep = yc_;
+        yc_ = c;
+        const double ms = time_it([&] { substage(1); });
+        snprintf(buf, sizeof buf, "yc=%d", c);
+        note(buf, ms);
+        if (ms < best_ms) best_ms = ms;
+        else yc_ = keep;
+      }
+      bm_ = 1;
+      for (int f = 0; f < F_; ++f) {
+        GNSS_HIP(hipMemcpyAsync(qb_[f], snap[2 * f], fbytes, hipMemcpyDeviceToDevice, s_));
+        GNSS_HIP(hipMemcpyAsync(dqb_[f], snap[2 * f + 1], fbytes, hipMemcpyDeviceToDevice, s_));
+      }
+      sync();
+    } catch (...) {
+      bm_ = 1;
+      (void)hipStreamSynchronize(s_);
+      for (void* v : snap) (void)hipFree(v);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      throw;
+    }
+    for (void* v : snap) (void)hipFree(v);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    timer_.enable(was_on, s_);
+    return rep + "}, \"choice\": \"" + tuning_string() + "\"}";
+  }
+
  private:
   // fused z variants built: FP32 0-7 (8 is 7 in FP32), FP64 0, 3, 6, 7 and 8
   static bool zvariant_ok(int v) {
@@ -906,12 +1026,15 @@ class Engine : public EngineBase {
     sync();
   }
 
+  // Physical work buffers of the unfused step, allocated on its first substage.  They are
+  // published only once every allocation succeeded (dalloc records each one for release()).
   void ensure_unfused_buffers() {
     if (!wp_.empty()) return;
-    wp_.assign(W_, nullptr);
-    nlp_.assign(F_, nullptr);
-    for (int w = 0; w < W_; ++w) dalloc(wp_[w], sizeof(T) * nph_);
-    for (int f = 0; f < F_; ++f) dalloc(nlp_[f], sizeof(T) * nph_);
+    std::vector<T*> wp(W_, nullptr), nlp(F_, nullptr);
+    for (int w = 0; w < W_; ++w) dalloc(wp[w], sizeof(T) * nph_);
+    for (int f = 0; f < F_; ++f) dalloc(nlp[f], sizeof(T) * nph_);
+    nlp_.swap(nlp);
+    wp_.swap(wp);
   }
 
   void release() {

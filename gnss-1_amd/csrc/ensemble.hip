@@ -75,17 +75,19 @@ GNSS_API int gnss_ensemble_create(int n, const gnss_params* members, gnss_ens** 
   try {
     for (int i = 0; i < n; ++i) {
       const gnss_params& p = members[i];
-      if (!gnss::supported_len(p.nx) || !gnss::supported_len(p.ny) || !gnss::supported_len(p.nz))
-        throw Error(GNSS_ERR_ARG, "member " + std::to_string(i) + ": unsupported grid");
-      if (!(p.re > 0 && p.dt > 0 && p.lx > 0 && p.ly > 0 && p.lz > 0) || p.n_scalars < 0 ||
-          p.n_scalars > GNSS_MAX_SCALARS)
-        throw Error(GNSS_ERR_ARG, "member " + std::to_string(i) + ": bad parameters");
+      const std::string bad = gnss::params_problem(p);
+      if (!bad.empty()) throw Error(GNSS_ERR_ARG, "member " + std::to_string(i) + ": " + bad);
+      // the unfused step allocates its physical buffers on first use, which a captured step
+      // cannot do
+      if (p.flags & GNSS_FLAG_UNFUSED)
+        throw Error(GNSS_ERR_UNSUPPORTED, "member " + std::to_string(i) +
+                                              ": GNSS_FLAG_UNFUSED is not available in an ensemble");
       e->p.push_back(p);
     }
     // groups of members that one batched engine can hold, in order of first appearance
     const char* env = getenv("GNSS_ENSEMBLE_BATCH");
     const bool batch = !(env && env[0] == '0');
-    const unsigned solo = GNSS_FLAG_COLLOID | GNSS_FLAG_MFMA_R16 | GNSS_FLAG_UNFUSED;
+    const unsigned solo = GNSS_FLAG_COLLOID | GNSS_FLAG_MFMA_R16;
     auto same_grid = [&](const gnss_params& a, const gnss_params& b) {
       return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.n_scalars == b.n_scalars &&
              a.precision == b.precision && a.flags == b.flags && a.lx == b.lx && a.ly == b.ly &&
@@ -148,6 +150,38 @@ GNSS_API int gnss_ensemble_destroy(gnss_ens* e) {
 
 GNSS_API const char* gnss_ensemble_last_error(const gnss_ens* e) { return e ? e->err.c_str() : g_ens_err.c_str(); }
 
+// Captures one RK3 step of every engine into the ensemble's graph (replacing any earlier one).
+static void ens_capture(gnss_ens* e) {
+  e->ready = false;
+  if (e->exec) {
+    GNSS_HIP(hipGraphExecDestroy(e->exec));
+    e->exec = nullptr;
+  }
+  if (e->graph) {
+    GNSS_HIP(hipGraphDestroy(e->graph));
+    e->graph = nullptr;
+  }
+  GNSS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+  try {
+    GNSS_HIP(hipEventRecord(e->fork, e->cap));
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      hipStream_t s = e->m[i]->stream();
+      GNSS_HIP(hipStreamWaitEvent(s, e->fork, 0));
+      e->m[i]->enqueue_rk3_step();
+      GNSS_HIP(hipEventRecord(e->done[i], s));
+      GNSS_HIP(hipStreamWaitEvent(e->cap, e->done[i], 0));
+    }
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(e->cap, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  GNSS_HIP(hipStreamEndCapture(e->cap, &e->graph));
+  GNSS_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  e->ready = true;
+}
+
 // Wavenumbers + jet initial condition of every member (seeds, theta, Re from each member's
 // parameters), then capture of the one-step graph.
 GNSS_API int gnss_ensemble_init(gnss_ens* e) {
@@ -156,33 +190,28 @@ GNSS_API int gnss_ensemble_init(gnss_ens* e) {
       x->init_wavenumbers();
       x->init_jet();
     }
-    if (e->exec) {
-      GNSS_HIP(hipGraphExecDestroy(e->exec));
-      e->exec = nullptr;
+    ens_capture(e);
+  });
+}
+
+// Tunes every batched engine of the ensemble on its own buffers (the "<key>b" entries of the
+// tuning table), keeps the choices and captures the step graph again with them.  The report is a
+// JSON list with one object per batched engine.  Needs gnss_ensemble_init first; the members'
+// states are restored after the timed substages.
+GNSS_API int gnss_ensemble_autotune(gnss_ens* e, int reps, char* report, int len) {
+  return eguard(e, [&] {
+    if (!e->ready) throw Error(GNSS_ERR_STATE, "call gnss_ensemble_init first");
+    std::string s = "[";
+    for (EngineBase* x : e->m) {
+      if (x->members() < 2) continue;
+      s += (s.size() > 1 ? ", " : "") + x->autotune(reps);
     }
-    if (e->graph) {
-      GNSS_HIP(hipGraphDestroy(e->graph));
-      e->graph = nullptr;
+    s += "]";
+    ens_capture(e);
+    if (report && len > 0) {
+      if ((int)s.size() + 1 > len) throw Error(GNSS_ERR_ARG, "report buffer too small");
+      memcpy(report, s.c_str(), s.size() + 1);
     }
-    GNSS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
-    try {
-      GNSS_HIP(hipEventRecord(e->fork, e->cap));
-      for (size_t i = 0; i < e->m.size(); ++i) {
-        hipStream_t s = e->m[i]->stream();
-        GNSS_HIP(hipStreamWaitEvent(s, e->fork, 0));
-        e->m[i]->enqueue_rk3_step();
-        GNSS_HIP(hipEventRecord(e->done[i], s));
-        GNSS_HIP(hipStreamWaitEvent(e->cap, e->done[i], 0));
-      }
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(e->cap, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
-    }
-    GNSS_HIP(hipStreamEndCapture(e->cap, &e->graph));
-    GNSS_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
-    e->ready = true;
   });
 }
 
@@ -213,6 +242,9 @@ GNSS_API int gnss_ensemble_get_field(gnss_ens* e, int i, int f, int space, doubl
     if (i < 0 || i >= (int)e->p.size() || !host) throw Error(GNSS_ERR_ARG, "member out of range");
     EngineBase* x = e->m[e->eng_of[i]];
     x->select_member(e->mem_of[i]);
+    // the physical buffers of a batched engine serve one member at a time: transform this
+    // member's state into them first
+    if (space == GNSS_PHYSICAL) x->inverse(f);
     x->get_field(f, space, host);
   });
 }

@@ -126,6 +126,7 @@ SIGNATURES = {
     "gnss_ensemble_step": (_I, [_P, _I]),
     "gnss_ensemble_stats": (_I, [_P, _I, ctypes.POINTER(StatsOut), _D, _D]),
     "gnss_ensemble_get_field": (_I, [_P, _I, _I, _I, _D]),
+    "gnss_ensemble_autotune": (_I, [_P, _I, ctypes.c_char_p, _I]),
 }
 
 _loaded: dict = {}
@@ -419,6 +420,21 @@ class Ensemble:
         self._check(self.lib.gnss_ensemble_get_field(self.ens, int(i), int(f), SPECTRAL,
                                                      a.ctypes.data_as(_D)))
         return a
+
+    def get_physical(self, i: int, f: int) -> np.ndarray:
+        """Member i's field f in physical space, [x][y][z], from its current spectral state."""
+        nx, ny, nz = self.cfgs[i].n
+        a = np.zeros((nx, ny, nz), dtype=np.float64)
+        self._check(self.lib.gnss_ensemble_get_field(self.ens, int(i), int(f), PHYSICAL,
+                                                     a.ctypes.data_as(_D)))
+        return a
+
+    def autotune(self, reps: int = 5) -> list:
+        """Tunes the batched engines (their "<key>b" table entries); returns one report per
+        batched engine."""
+        buf = ctypes.create_string_buffer(1 << 20)
+        self._check(self.lib.gnss_ensemble_autotune(self.ens, int(reps), buf, len(buf)))
+        return json.loads(buf.value.decode())
 
 
 # The five workloads the engine is measured on (synthetic seeded jets, L = 2 pi).

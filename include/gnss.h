@@ -221,6 +221,8 @@ int gnss_set_tuning(gnss_ctx* ctx, const char* spec);
  *   launch (GNSS_ENSEMBLE_BATCH=0 in the environment: one engine per member); one RK3 step of the
  *   whole ensemble is one hipGraph launch.  Member indices follow the order given here. */
 typedef struct gnss_ens gnss_ens;
+/* Members are checked as gnss_create checks them; GNSS_FLAG_UNFUSED members are refused
+ *   (GNSS_ERR_UNSUPPORTED): the captured step cannot allocate their physical buffers. */
 int gnss_ensemble_create(int n, const gnss_params* members, gnss_ens** out);
 int gnss_ensemble_destroy(gnss_ens* ens);
 const char* gnss_ensemble_last_error(const gnss_ens* ens);
@@ -228,7 +230,15 @@ int gnss_ensemble_init(gnss_ens* ens);
 int gnss_ensemble_step(gnss_ens* ens, int nsteps);
 int gnss_ensemble_stats(gnss_ens* ens, int member, gnss_stats_out* out, double* yprof,
                         double* spectrum);
+/* Member `member`'s field as gnss_get_field gives it; GNSS_PHYSICAL inverse-transforms that
+ *   member's current spectral state first (an ensemble has no gnss_inverse of its own). */
 int gnss_ensemble_get_field(gnss_ens* ens, int member, int field, int space, double* host);
+/* ≈ plan1dFFT / plan2dFFT's planner choice (fftfuncs.cu:133-231), for the batched engines: times
+ *   each batched engine's pass variants over whole substages of all its members, keeps the
+ *   fastest (the "<key>b" entries of the tuning table) and captures the step graph again.  Needs
+ *   gnss_ensemble_init first; the members' states are restored.  report (may be NULL) receives a
+ *   JSON list with one object per batched engine. */
+int gnss_ensemble_autotune(gnss_ens* ens, int reps, char* report, int len);
 
 #ifdef __cplusplus
 }

@@ -1325,5 +1325,15 @@ EXPORT int gnss_ensemble_stats(gnss_ens* e, int i, gnss_stats_out* o, double* yp
 }
 EXPORT int gnss_ensemble_get_field(gnss_ens* e, int i, int f, int s, double* h) {
   if (!e || i < 0 || i >= e->n) return GNSS_ERR_ARG;
+  if (s == GNSS_PHYSICAL) { /* the member's current state, as the engine gives it */
+    int rc = gnss_inverse(e->m[i], f);
+    if (rc) return rc;
+  }
   return gnss_get_field(e->m[i], f, s, h);
+}
+EXPORT int gnss_ensemble_autotune(gnss_ens* e, int r, char* b, int n) {
+  (void)r; (void)b; (void)n;
+  if (!e) return GNSS_ERR_ARG;
+  snprintf(e->err, sizeof e->err, "engine only");
+  return GNSS_ERR_UNSUPPORTED;
 }

@@ -167,3 +167,59 @@ def test_bench_calls_report_rates(g, engine):
     assert f > 0 and i > 0
     with pytest.raises(g.GnssError):
         s.bench_copy(0)
+
+
+def test_ensemble_validates_members(g, engine):
+    """gnss_ensemble_create checks members as gnss_create does, and refuses the unfused step (its
+    physical buffers would be allocated inside the captured step)."""
+    base = g.ensemble_configs(counts=((32, 2),), precision=g.FP64)
+    for bad, code in [(dict(sc=(0.0,)), 1), (dict(precision=7), 1), (dict(flags=g.FLAG_UNFUSED), 6)]:
+        cfgs = [base[0], g.Config(**{**base[1].__dict__, **bad})]
+        with pytest.raises(g.GnssError) as e:
+            g.Ensemble(cfgs, engine)
+        assert e.value.code == code, bad
+
+
+def test_ensemble_physical_field_is_the_members(g, engine):
+    """GNSS_PHYSICAL through the ensemble is the selected member's current state transformed, not
+    whatever another member's call left in the batched engine's shared physical buffers."""
+    cfgs = g.ensemble_configs(counts=((32, 3),), precision=g.FP64)
+    ens = g.Ensemble(cfgs, engine)
+    ens.init()
+    ens.step(2)
+    ens.stats(2)  # leaves member 2's physical fields in the shared buffers
+    for i in (0, 1, 2, 0):
+        s = g.Solver(cfgs[i], engine).setup()
+        s.step(2)
+        for f in range(4):
+            s.inverse(f)
+            a, b = ens.get_physical(i, f), s.get_physical(f)
+            assert rel(a, b) <= 1e-14, (i, f)
+        s.close()
+    ens.close()
+
+
+def test_ensemble_autotune_tunes_batched_engines(g, engine):
+    """gnss_ensemble_autotune times the batched engines' variants (their "<key>b" entries), keeps
+    the states, captures the step again with the choices, and the members still equal their
+    stand-alone runs."""
+    cfgs = g.ensemble_configs(counts=((32, 4), (48, 3), (64, 1)))
+    ens = g.Ensemble(cfgs, engine)
+    ens.init()
+    ens.step(1)
+    before = [ens.get_spectral(i, 0) for i in range(len(cfgs))]
+    reps = ens.autotune(1)
+    assert sorted(r["key"] for r in reps) == ["f32 32 32 32 1b", "f32 48 48 48 1b"]
+    for r in reps:
+        assert {"zf=7", "xc=16 cp=1 cb=1 lp=1", "xc=8 cp=0 cb=0 lp=2"} <= set(r["candidates"]), r["key"]
+        assert r["members"] == (4 if r["key"].startswith("f32 32") else 3)
+    for i in range(len(cfgs)):
+        assert ens.get_spectral(i, 0).tobytes() == before[i].tobytes(), i
+    ens.step(2)
+    for i in (0, 3, 4, 7):
+        s = g.Solver(cfgs[i], engine).setup()
+        s.step(3)
+        for f in range(4):
+            assert rel(ens.get_spectral(i, f), s.get_spectral(f)) <= 2e-6, (i, f)
+        s.close()
+    ens.close()

@@ -35,3 +35,28 @@ def test_measured_workloads_are_tuned():
                 "f32 64 64 64 1", "f32 96 96 96 1", "f32 128 128 128 1", "f32 512 512 512 2c",
                 "f64 512 512 512 2c"]:
         assert key in t, key
+
+
+def test_autotuner_merges_into_the_table(tmp_path):
+    """Re-tuning replaces only the keys it tunes: every other entry (the batched "<key>b" ones
+    included) survives, and the untuned row padding kp keeps its table value."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("autotune", os.path.join(PKG, "autotune.py"))
+    at = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(at)
+    path = str(tmp_path / "t.json")
+    old = json.load(open(TABLE))
+    at.write_table(path, old)
+    table = at.read_table(path)
+    assert table == old
+    key = "f32 512 512 512 1"
+    table[key] = dict(table[key], kp=2)
+    rep = {"key": key, "choice": "cx=8 cy=16 zf=6 xf=4 cp=1 xc=16 zp=1 kp=0 cb=1 lp=1 yc=0"}
+    table[key] = at.choice_of(rep, table[key])
+    at.write_table(path, table)
+    new = json.load(open(path))
+    assert new[key]["cx"] == 8 and new[key]["zf"] == 6 and new[key]["kp"] == 2
+    for k in old:
+        if k != key:
+            assert new[k] == old[k], k
+    assert any(k.endswith("b") for k in new)
