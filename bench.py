@@ -100,25 +100,57 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(cfg, threads, steps=3):
-    """Bounded sample: `steps` FP64 RK3 steps of the same workload on the oracle (FFTW3 API)."""
+def cpu_baseline(cfg, threads, steps=3, one_core_steps=1):
+    """Bounded sample: `steps` FP64 RK3 steps of the same workload on the oracle (FFTW3 API) with
+    `threads` threads, then `one_core_steps` steps on one thread (the single-core figure)."""
     lib = gnss.load(ORACLE_LIB)
     lib.gnss_oracle_set_threads.argtypes = [gnss.ctypes.c_int]
     lib.gnss_oracle_set_backend.argtypes = [gnss.ctypes.c_int]
     lib.gnss_oracle_backend_name.restype = gnss.ctypes.c_char_p
     lib.gnss_oracle_max_threads.restype = gnss.ctypes.c_int
-    lib.gnss_oracle_set_threads(threads)
     backend = "fftw3-api" if lib.gnss_oracle_set_backend(1) == 0 else "builtin"
-    c64 = gnss.Config(**{**cfg.__dict__, "precision": gnss.FP64})
-    o = gnss.Solver(c64, lib).setup()
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
     name = lib.gnss_oracle_backend_name().decode()
-    o.close()
-    return {"value": round(steps / dt, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} FP64 RK3 steps of the same 512^3 workload on the oracle ({backend}: {name}; "
-                      f"{threads} threads of {os.cpu_count()} on {cpu_model()}); {dt:.1f} s"}
+    c64 = gnss.Config(**{**cfg.__dict__, "precision": gnss.FP64})
+    n = cfg.n
+
+    def timed_steps(nthreads, k):
+        lib.gnss_oracle_set_threads(nthreads)
+        o = gnss.Solver(c64, lib).setup()
+        t0 = time.perf_counter()
+        o.step(k)
+        dt = time.perf_counter() - t0
+        o.close()
+        return dt
+
+    dt = timed_steps(threads, steps)
+    out = {"value": round(steps / dt, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} FP64 RK3 steps of the same {n[0]}^3 workload on the oracle ({backend}: {name}; "
+                     f"{threads} threads of {os.cpu_count()} on {cpu_model()}); {dt:.1f} s"}
+    if one_core_steps > 0:
+        d1 = timed_steps(1, one_core_steps)
+        out["one_core"] = {"value": round(one_core_steps / d1, 5), "unit": "steps/s", "cores": 1,
+                           "sample": f"{one_core_steps} FP64 RK3 step(s) of the same workload, 1 thread; {d1:.1f} s"}
+    lib.gnss_oracle_set_threads(threads)
+    return out
+
+
+def step_traffic(rep, steps, ms_per_step, cfg):
+    """Algorithmic HBM bytes of one timed RK3 step (the sum of the kernels' per-launch byte counts:
+    the pruned, fused passes the step runs), the rate they imply, and the unfused, unpruned
+    per-step figure B_step = 3 [(9 + 4S) B_fft + 5 (3 + S) C] for comparison."""
+    b = 8 if cfg.precision == gnss.FP64 else 4
+    n, S = cfg.n, len(cfg.sc)
+    modes = n[0] * n[1] * (n[2] // 2 + 1)
+    b_fft = 3 * 2 * modes * 2 * b
+    c = modes * 2 * b
+    b_step = 3 * ((9 + 4 * S) * b_fft + 5 * (3 + S) * c)
+    step_bytes = sum(v["bytes"] for v in rep.values()) / steps
+    gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+    return {"step_bytes": step_bytes, "step_gbs": round(gbs, 1), "step_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "b_step_unfused": b_step,
+            "note": "step_bytes counts what the fused, 2/3-pruned passes move (C2R + products + R2C in one "
+                    "z pass, known-zero modes neither loaded nor stored); b_step_unfused is the same step as "
+                    "separate full transforms and pointwise passes"}
 
 
 def main():
@@ -190,6 +222,8 @@ def main():
         "kernels_f32": rep,
         "kernels_f64": rep64,
         "roofline": roof,
+        "step_traffic": {"f32": step_traffic(rep, args.steps, 1000.0 * t32 / args.steps, cfg32),
+                         "f64": step_traffic(rep64, args.steps, 1000.0 * t64 / args.steps, cfg64)},
     }
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)

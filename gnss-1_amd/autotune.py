@@ -13,7 +13,7 @@ raw candidate timings go to --report (default gpurun_out/autotune_report.json), 
 table.
 
 usage: python gnss-1_amd/autotune.py [--reps 5] [--out gnss-1_amd/tuning/gnss_tuning.json]
-                                     [--big] [--ensemble | --ensemble-only]
+                                     [--big] [--only KEY ...] [--ensemble | --ensemble-only]
 """
 import argparse
 import json
@@ -66,8 +66,16 @@ def choice_of(rep, old):
     return ch
 
 
-def tune_single(reps, big, table, reports):
+def key_of(cfg):
+    """The engine's tuning key of a workload: precision nx ny nz S[c]."""
+    return (f"{'f32' if cfg.precision == gnss.FP32 else 'f64'} {cfg.n[0]} {cfg.n[1]} {cfg.n[2]} {len(cfg.sc)}"
+            f"{'c' if cfg.flags & gnss.FLAG_COLLOID else ''}")
+
+
+def tune_single(reps, big, table, reports, only=None):
     for cfg in cases(big):
+        if only and key_of(cfg) not in only:
+            continue
         t0 = time.time()
         s = gnss.Solver(cfg).setup()
         rep = s.autotune(reps)
@@ -97,10 +105,12 @@ def main():
     ap.add_argument("--big", action="store_true", help="also tune the 1024^3 FP32 workload")
     ap.add_argument("--ensemble", action="store_true", help="also tune the batched ensemble engines")
     ap.add_argument("--ensemble-only", action="store_true", help="tune only the batched ensemble engines")
+    ap.add_argument("--only", action="append", default=[],
+                    help="tune only this workload key (e.g. 'f32 512 512 512 1'); repeatable")
     args = ap.parse_args()
     table, reports = read_table(args.out), []
     if not args.ensemble_only:
-        tune_single(args.reps, args.big, table, reports)
+        tune_single(args.reps, args.big, table, reports, set(args.only))
     if args.ensemble or args.ensemble_only:
         # the batched engines read the table's single-context entries first: write them before
         write_table(args.out, table)

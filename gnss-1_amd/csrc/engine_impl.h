@@ -783,9 +783,10 @@ class Engine : public EngineBase {
         return (double)ms / reps;
       };
       bool first = true;
-      auto note = [&](const char* what, double ms) {
-        snprintf(buf, sizeof buf, "%s\"%s\": %.5f", first ? "" : ", ", what, ms);
-        rep += buf;
+      auto note = [&](const char* what, double ms) {  // `what` may point into buf
+        char line[200];
+        snprintf(line, sizeof line, "%s\"%s\": %.5f", first ? "" : ", ", what, ms);
+        rep += line;
         first = false;
       };
       {

@@ -3,6 +3,8 @@
 //   restart from raw field files) -> stats(0), slices(0) -> loop { step; stats every n_stats;
 //   2D slices every n_vis; checkpoint every n_checkpoint } -> free.
 // The reference hard-codes its configuration in dnsparams.h; here it comes from the command line.
+// The driver only calls include/gnss.h, so it runs on any library implementing that ABI (built
+// against the MI355X engine here).
 //
 // usage: gnss_main [--n N | --nx X --ny Y --nz Z] [--re R] [--sc S ...] [--theta T] [--dt DT]
 //                  [--steps K] [--stats-every M] [--kmax K] [--amp A] [--seed S] [--fp32]
@@ -30,7 +32,8 @@ static void check(int rc, gnss_ctx* c, const char* what) {
 }
 
 static void print_stats(gnss_ctx* ctx, const gnss_stats_out& s, int S, double ms, double iso) {
-  printf("step %6lld  t %9.5f  KE %.10e  eps %.6e  chi0 %.6e  CFL %.4f  div %.2e  (%.3f ms/step)",
+  // full precision, so that runs on different implementations of the ABI can be compared
+  printf("step %6lld  t %9.5f  KE %.16e  eps %.16e  chi0 %.16e  CFL %.16e  div %.2e  (%.3f ms/step)",
          (long long)s.step, s.t, s.ke, s.epsilon, S > 0 ? s.chi[0] : 0.0, s.cfl, s.divergence, ms);
   if (S > 0 && !std::isnan(iso)) {
     // scalar interface area (the reference's area_scalar, statistics.cu:2132)

@@ -805,10 +805,14 @@ static void rk_substage(gnss_ctx* c, int s) {
       }
       for (int f = 0; f < c->F; ++f) {
         const double nu = f < 3 ? c->nu : c->diff[f - 3];
-        const double e = exp(-nu * k2 * tau);
-        cplx d = (s == 0 ? 0 : a * c->dq[f][i]) + dt * rhs[f];
-        c->q[f][i] = e * (c->q[f][i] + b * d);
-        c->dq[f][i] = e * d;
+        /* E = exp(-nu k^2 tau) applied as 1 - eps, eps = -expm1(-nu k^2 tau): the same factor,
+           formed so that an E close to 1 carries no rounding of its own.
+           E (q + b d) = q + (b d - eps (q + b d)): the state is rounded once per substage. */
+        const double eps = -expm1(-nu * k2 * tau);
+        const cplx d = (s == 0 ? 0 : a * c->dq[f][i]) + dt * rhs[f];
+        const cplx qf = c->q[f][i];
+        c->q[f][i] = qf + (b * d - eps * (qf + b * d));
+        c->dq[f][i] = d - eps * d;
       }
     }
   }
