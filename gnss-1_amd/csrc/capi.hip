@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 
 #include <dlfcn.h>
@@ -20,10 +21,39 @@ using gnss::Error;
 struct gnss_ctx {
   EngineBase* eng = nullptr;
   gnss_params p{};
+  int dev = 0;  // device current at gnss_create (the engine's stream lives there)
   std::string err;
 };
 
 static thread_local std::string g_create_err;
+
+// ---------------------------------------------------------------------------------------------
+// Fixed device scratch of gnss_stats_noalloc: one copy per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (on the host it names a stand-in, not
+// device memory): gnss_create resolves the device address once per device with
+// hipGetSymbolAddress while that device is current, and every use goes through that address.
+// ---------------------------------------------------------------------------------------------
+#define GNSS_STATS_SCRATCH_DOUBLES (5u << 20)  // 40 MiB
+static __device__ __align__(256) double scratch[GNSS_STATS_SCRATCH_DOUBLES];
+static constexpr int GNSS_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[GNSS_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device `dev` (current), resolved on the first call there.
+static void* stats_scratch(int dev) {
+  if (dev < 0 || dev >= GNSS_MAX_DEVICES) throw Error(GNSS_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      throw Error(GNSS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
 
 namespace gnss {
 
@@ -145,6 +175,8 @@ GNSS_API int gnss_create(const gnss_params* p, gnss_ctx** out) {
   gnss_ctx* c = new gnss_ctx();
   c->p = *p;
   try {
+    GNSS_HIP(hipGetDevice(&c->dev));
+    (void)stats_scratch(c->dev);  // once per device, with the context's device current
     c->eng = p->precision == GNSS_FP64 ? gnss::make_engine_f64(*p) : gnss::make_engine_f32(*p);
   } catch (const Error& e) {
     g_create_err = e.what();
@@ -189,6 +221,27 @@ GNSS_API int gnss_dealias(gnss_ctx* c) { return guard(c, [&] { c->eng->dealias()
 GNSS_API int gnss_step(gnss_ctx* c, int n) { return guard(c, [&] { c->eng->step(n); }); }
 GNSS_API int gnss_stats(gnss_ctx* c, gnss_stats_out* o, double* yprof, double* spectrum) {
   return guard(c, [&] { c->eng->stats(o, yprof, spectrum); });
+}
+// gnss_stats without any allocation: the partial sums live in the fixed per-device scratch above
+// (resolved by gnss_create), so a caller that cannot allocate can take statistics.  Every call on
+// one GPU shares that scratch: calls that overlap on one GPU (several threads or contexts) race.
+// Each call writes every partial before reading it, so nothing carries over between calls.
+GNSS_API int gnss_stats_noalloc(gnss_ctx* c, gnss_stats_out* o, double* yprof, double* spectrum) {
+  return guard(c, [&] {
+    const size_t need = c->eng->stats_scratch_doubles();
+    if (need > GNSS_STATS_SCRATCH_DOUBLES)
+      throw Error(GNSS_ERR_ALLOC, "gnss_stats_noalloc: this grid's statistics need " + std::to_string(need * 8 >> 10) +
+                                      " KiB of scratch, above the fixed " +
+                                      std::to_string((size_t)GNSS_STATS_SCRATCH_DOUBLES * 8 >> 20) +
+                                      " MiB; use gnss_stats");
+    void* ptr = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      ptr = g_scratch_ptr[c->dev];
+    }
+    if (!ptr) throw Error(GNSS_ERR_STATE, "gnss_stats_noalloc: scratch of this device not resolved");
+    c->eng->stats_noalloc(static_cast<double*>(ptr), o, yprof, spectrum);
+  });
 }
 GNSS_API int gnss_get_field(gnss_ctx* c, int f, int space, double* host) {
   if (!host) return GNSS_ERR_ARG;

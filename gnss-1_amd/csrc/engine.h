@@ -65,6 +65,11 @@ class EngineBase {
   virtual void dealias() = 0;
   virtual void step(int nsteps) = 0;
   virtual void stats(gnss_stats_out* out, double* yprof, double* spectrum) = 0;
+  // The same statistics with every partial sum in `scratch` (device memory the caller owns, at
+  // least stats_scratch_doubles() doubles): no allocation on the device or the host, no timer
+  // events; results bitwise equal to stats().
+  virtual size_t stats_scratch_doubles() const = 0;
+  virtual void stats_noalloc(double* scratch, gnss_stats_out* out, double* yprof, double* spectrum) = 0;
   virtual void get_field(int field, int space, double* host) = 0;
   virtual void set_field(int field, int space, const double* host) = 0;
   virtual void sizes(gnss_sizes* out) const = 0;

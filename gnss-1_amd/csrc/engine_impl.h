@@ -273,33 +273,8 @@ class Engine : public EngineBase {
 
   void stats(gnss_stats_out* o, double* yprof, double* spectrum) override {
     need_k();
-    const double nn = (double)nx_ * ny_ * nz_;
-    const double inn2 = 1.0 / (nn * nn);
     const int nb = n_bins();
-    // spectral sums per kx plane
-    timed("k_spec_sums", (double)F_ * nx_ * ny_ * nzh_ * sizeof(C), [&] {
-      hipLaunchKernelGGL(k_spec_sums<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, ptrs(q_), dkx_,
-                         dky_, dkz_, S_, inn2, d_part_);
-    });
-    ShellArgs sa;
-    sa.equal_box = (p_.lx == p_.ly && p_.ly == p_.lz) ? 1 : 0;
-    const double lm = std::max(p_.lx, std::max(p_.ly, p_.lz));
-    sa.fx = lm / p_.lx; sa.fy = lm / p_.ly; sa.fz = lm / p_.lz;
-    timed("k_spectrum", (double)F_ * nx_ * ny_ * nzh_ * sizeof(C), [&] {
-      hipLaunchKernelGGL(k_spectrum<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, ptrs(q_), S_, nb, sa,
-                         inn2, d_spec_);
-    });
-    // physical fields
-    for (int f = 0; f < F_; ++f) {
-      GNSS_HIP(hipMemcpyAsync(ws_[0], q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
-      inv3d(ws_[0], phys_[f], false);
-    }
-    RPtrs<T> pp{};
-    for (int f = 0; f < F_; ++f) pp.p[f] = phys_[f];
-    timed("k_plane_stats", (double)F_ * nph_ * sizeof(T) * 2, [&] {
-      hipLaunchKernelGGL(k_plane_stats<T>, dim3(ny_), dim3(STAT_THREADS), 0, s_, g_, pp, S_,
-                         nx_ / p_.lx, ny_ / p_.ly, nz_ / p_.lz, d_prof_, d_cmax_);
-    });
+    stats_kernels(d_part_, d_spec_, d_prof_, d_cmax_);
     std::vector<double> part((size_t)nx_ * SPEC_NQ), spec((size_t)nx_ * (1 + S_) * nb),
         prof((size_t)(7 + 3 * S_) * ny_), cm((size_t)2 * ny_);
     GNSS_HIP(hipMemcpyAsync(part.data(), d_part_, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s_));
@@ -307,34 +282,14 @@ class Engine : public EngineBase {
     GNSS_HIP(hipMemcpyAsync(prof.data(), d_prof_, sizeof(double) * prof.size(), hipMemcpyDeviceToHost, s_));
     GNSS_HIP(hipMemcpyAsync(cm.data(), d_cmax_, sizeof(double) * cm.size(), hipMemcpyDeviceToHost, s_));
     sync();
-    double sum[SPEC_NQ] = {0};
+    double sum[SPEC_NQ + 2] = {0};
     for (int x = 0; x < nx_; ++x)
       for (int qq = 0; qq < SPEC_NQ; ++qq) sum[qq] += part[(size_t)x * SPEC_NQ + qq];
-    const double nu = 1.0 / p_.re;
-    if (o) {
-      memset(o, 0, sizeof(*o));
-      o->t = t_;
-      o->step = step_;
-      o->ke = 0.5 * sum[0];
-      o->vrms = sqrt(sum[0] / 3.0);
-      o->epsilon = nu * sum[1];
-      o->enstrophy = 0.5 * sum[1];
-      o->lint = o->ke > 0 ? 3.0 * 3.14159265358979323846 / 4.0 * sum[2] / o->ke : 0.0;
-      o->lambda = o->epsilon > 0 ? sqrt(15.0 * nu * o->vrms * o->vrms / o->epsilon) : 0.0;
-      o->eta = o->epsilon > 0 ? sqrt(sqrt(nu * nu * nu / o->epsilon)) : 0.0;
-      o->divergence = sum[1] > 0 ? sqrt(sum[3] / sum[1]) : 0.0;
-      for (int s = 0; s < S_; ++s) {
-        o->chi[s] = 2.0 * (nu / p_.sc[s]) * sum[4 + 2 * s];
-        o->scalar_var[s] = sum[5 + 2 * s];
-      }
-      double cmx = 0, um = 0;
-      for (int y = 0; y < ny_; ++y) {
-        cmx = std::max(cmx, cm[2 * y]);
-        um = std::max(um, cm[2 * y + 1]);
-      }
-      o->cfl = p_.dt * cmx;
-      o->umax = um;
+    for (int y = 0; y < ny_; ++y) {
+      sum[SPEC_NQ] = std::max(sum[SPEC_NQ], cm[2 * y]);
+      sum[SPEC_NQ + 1] = std::max(sum[SPEC_NQ + 1], cm[2 * y + 1]);
     }
+    stats_finish(sum, o);
     if (yprof) memcpy(yprof, prof.data(), sizeof(double) * prof.size());
     if (spectrum) {
       for (int s = 0; s <= S_; ++s)
@@ -344,6 +299,108 @@ class Engine : public EngineBase {
           spectrum[(size_t)s * nb + b] = acc;
         }
     }
+  }
+
+  // Scratch layout of stats_noalloc, regions 256-byte aligned: per-plane spectral sums, per-plane
+  // shell sums, profiles, per-y maxima, then the reduced results (SPEC_NQ sums, 2 maxima, the
+  // spectrum).
+  struct StatScratch {
+    size_t part, spec, prof, cmax, fin, total;
+  };
+  StatScratch stat_scratch_layout() const {
+    auto up = [](size_t n) { return (n + 31) / 32 * 32; };
+    const size_t nb = n_bins();
+    StatScratch l;
+    l.part = 0;
+    l.spec = l.part + up((size_t)nx_ * SPEC_NQ);
+    l.prof = l.spec + up((size_t)nx_ * (1 + S_) * nb);
+    l.cmax = l.prof + up((size_t)(7 + 3 * S_) * ny_);
+    l.fin = l.cmax + up((size_t)2 * ny_);
+    l.total = l.fin + up(SPEC_NQ + 2 + (size_t)(1 + S_) * nb);
+    return l;
+  }
+  size_t stats_scratch_doubles() const override { return stat_scratch_layout().total; }
+
+  // Every partial the kernels read is written earlier in the same call, so the scratch needs no
+  // clearing; the sums over planes run on the device in the order stats() sums them on the host.
+  void stats_noalloc(double* scratch, gnss_stats_out* o, double* yprof, double* spectrum) override {
+    need_k();
+    const int nb = n_bins();
+    const StatScratch l = stat_scratch_layout();
+    double* fin = scratch + l.fin;
+    pause_timer_ = true;
+    try {
+      stats_kernels(scratch + l.part, scratch + l.spec, scratch + l.prof, scratch + l.cmax);
+    } catch (...) {
+      pause_timer_ = false;
+      throw;
+    }
+    pause_timer_ = false;
+    hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(STAT_THREADS), 0, s_, scratch + l.part, scratch + l.spec,
+                       scratch + l.cmax, nx_, ny_, (1 + S_) * nb, fin);
+    check_launch("k_stats_reduce");
+    double sum[SPEC_NQ + 2];
+    GNSS_HIP(hipMemcpyAsync(sum, fin, sizeof(sum), hipMemcpyDeviceToHost, s_));
+    if (yprof)
+      GNSS_HIP(hipMemcpyAsync(yprof, scratch + l.prof, sizeof(double) * (7 + 3 * S_) * ny_, hipMemcpyDeviceToHost, s_));
+    if (spectrum)
+      GNSS_HIP(hipMemcpyAsync(spectrum, fin + SPEC_NQ + 2, sizeof(double) * (1 + S_) * nb, hipMemcpyDeviceToHost, s_));
+    sync();
+    stats_finish(sum, o);
+  }
+
+  // Statistics kernels: spectral sums and shell sums per kx plane, the physical fields, then the
+  // planar averages and maxima per y.
+  void stats_kernels(double* part, double* spec, double* prof, double* cmax) {
+    const double nn = (double)nx_ * ny_ * nz_;
+    const double inn2 = 1.0 / (nn * nn);
+    const int nb = n_bins();
+    timed("k_spec_sums", (double)F_ * nx_ * ny_ * nzh_ * sizeof(C), [&] {
+      hipLaunchKernelGGL(k_spec_sums<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, ptrs(q_), dkx_,
+                         dky_, dkz_, S_, inn2, part);
+    });
+    ShellArgs sa;
+    sa.equal_box = (p_.lx == p_.ly && p_.ly == p_.lz) ? 1 : 0;
+    const double lm = std::max(p_.lx, std::max(p_.ly, p_.lz));
+    sa.fx = lm / p_.lx; sa.fy = lm / p_.ly; sa.fz = lm / p_.lz;
+    timed("k_spectrum", (double)F_ * nx_ * ny_ * nzh_ * sizeof(C), [&] {
+      hipLaunchKernelGGL(k_spectrum<T>, dim3(nx_), dim3(STAT_THREADS), 0, s_, g_, ptrs(q_), S_, nb, sa,
+                         inn2, spec);
+    });
+    for (int f = 0; f < F_; ++f) {
+      GNSS_HIP(hipMemcpyAsync(ws_[0], q_[f], sizeof(C) * nsp_, hipMemcpyDeviceToDevice, s_));
+      inv3d(ws_[0], phys_[f], false);
+    }
+    RPtrs<T> pp{};
+    for (int f = 0; f < F_; ++f) pp.p[f] = phys_[f];
+    timed("k_plane_stats", (double)F_ * nph_ * sizeof(T) * 2, [&] {
+      hipLaunchKernelGGL(k_plane_stats<T>, dim3(ny_), dim3(STAT_THREADS), 0, s_, g_, pp, S_,
+                         nx_ / p_.lx, ny_ / p_.ly, nz_ / p_.lz, prof, cmax);
+    });
+  }
+
+  // Integral statistics from the summed spectral partials sum[0..SPEC_NQ) and the maxima of the
+  // CFL sum and of |u_i| (sum[SPEC_NQ], sum[SPEC_NQ + 1]).
+  void stats_finish(const double* sum, gnss_stats_out* o) const {
+    if (!o) return;
+    const double nu = 1.0 / p_.re;
+    memset(o, 0, sizeof(*o));
+    o->t = t_;
+    o->step = step_;
+    o->ke = 0.5 * sum[0];
+    o->vrms = sqrt(sum[0] / 3.0);
+    o->epsilon = nu * sum[1];
+    o->enstrophy = 0.5 * sum[1];
+    o->lint = o->ke > 0 ? 3.0 * 3.14159265358979323846 / 4.0 * sum[2] / o->ke : 0.0;
+    o->lambda = o->epsilon > 0 ? sqrt(15.0 * nu * o->vrms * o->vrms / o->epsilon) : 0.0;
+    o->eta = o->epsilon > 0 ? sqrt(sqrt(nu * nu * nu / o->epsilon)) : 0.0;
+    o->divergence = sum[1] > 0 ? sqrt(sum[3] / sum[1]) : 0.0;
+    for (int s = 0; s < S_; ++s) {
+      o->chi[s] = 2.0 * (nu / p_.sc[s]) * sum[4 + 2 * s];
+      o->scalar_var[s] = sum[5 + 2 * s];
+    }
+    o->cfl = p_.dt * sum[SPEC_NQ];
+    o->umax = sum[SPEC_NQ + 1];
   }
 
   void get_field(int f, int space, double* host) override {
@@ -943,10 +1000,11 @@ class Engine : public EngineBase {
     if (e != hipSuccess) throw Error(GNSS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
   }
   template <typename Fn> void timed(const char* name, double bytes, Fn&& fn) {
-    if (timer_.on()) timer_.begin(name, bytes);
+    const bool on = timer_.on() && !pause_timer_;
+    if (on) timer_.begin(name, bytes);
     fn();
     check_launch(name);
-    if (timer_.on()) timer_.end();
+    if (on) timer_.end();
   }
 
   template <typename P> void* dalloc(P*& ptr, size_t bytes) {
@@ -1842,7 +1900,7 @@ class Engine : public EngineBase {
   void* scratch_ = nullptr;
   size_t scratch_n_ = 0;
   std::vector<void*> allocs_;
-  bool have_k_ = false, have_ic_ = false;
+  bool have_k_ = false, have_ic_ = false, pause_timer_ = false;
   double t_ = 0;
   int64_t step_ = 0;
   KernelTimer timer_;

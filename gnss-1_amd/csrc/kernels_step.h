@@ -854,6 +854,30 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   }
 }
 
+// The reductions of the allocation-free statistics call, on the device and in the order the host
+// sums the same partials for gnss_stats (so both give the same bits): fin[q] = sum over x of
+// part[x][q] (q < SPEC_NQ); fin[SPEC_NQ], fin[SPEC_NQ + 1] = max over y of the CFL sum and of
+// max |u_i|; fin[SPEC_NQ + 2 + j] = sum over x of spec[x][j] (j < nspec shell entries).
+static __global__ void __launch_bounds__(STAT_THREADS)
+k_stats_reduce(const double* __restrict__ part, const double* __restrict__ spec, const double* __restrict__ cmax,
+               int nx, int ny, int nspec, double* __restrict__ fin) {
+  const int t = threadIdx.x;
+  if (t < SPEC_NQ) {
+    double acc = 0.0;
+    for (int x = 0; x < nx; ++x) acc += part[(long long)x * SPEC_NQ + t];
+    fin[t] = acc;
+  } else if (t < SPEC_NQ + 2) {
+    double m = 0.0;
+    for (int y = 0; y < ny; ++y) m = fmax(m, cmax[2 * y + (t - SPEC_NQ)]);
+    fin[t] = m;
+  }
+  for (int j = t; j < nspec; j += STAT_THREADS) {
+    double acc = 0.0;
+    for (int x = 0; x < nx; ++x) acc += spec[(long long)x * nspec + j];
+    fin[SPEC_NQ + 2 + j] = acc;
+  }
+}
+
 // Plain copy (bandwidth reference): block b copies the contiguous chunk [b chunk, (b + 1) chunk)
 // of 16-byte elements with four nontemporal loads in flight per thread before the stores.  Of the
 // copy forms measured on MI355X (grid-stride or chunked, 2-8 loads in flight, cached or

@@ -105,6 +105,7 @@ SIGNATURES = {
     "gnss_dealias": (_I, [_P]),
     "gnss_step": (_I, [_P, _I]),
     "gnss_stats": (_I, [_P, ctypes.POINTER(StatsOut), _D, _D]),
+    "gnss_stats_noalloc": (_I, [_P, ctypes.POINTER(StatsOut), _D, _D]),
     "gnss_get_field": (_I, [_P, _I, _I, _D]),
     "gnss_set_field": (_I, [_P, _I, _I, _D]),
     "gnss_get_time": (_I, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
@@ -257,13 +258,14 @@ class Solver:
     def inverse(self, f: int):
         self._check(self.lib.gnss_inverse(self.ctx, int(f)))
 
-    def stats(self, want_prof: bool = True, want_spec: bool = True):
+    def stats(self, want_prof: bool = True, want_spec: bool = True, noalloc: bool = False):
+        """gnss_stats (noalloc: gnss_stats_noalloc, the allocation-free form)."""
         o = StatsOut()
         yp = np.zeros((self.sizes.n_prof, self.ny)) if want_prof else None
         sp = np.zeros((1 + self.S, self.sizes.n_bins)) if want_spec else None
-        self._check(self.lib.gnss_stats(self.ctx, ctypes.byref(o),
-                                        _dptr(yp) if yp is not None else None,
-                                        _dptr(sp) if sp is not None else None))
+        fn = self.lib.gnss_stats_noalloc if noalloc else self.lib.gnss_stats
+        self._check(fn(self.ctx, ctypes.byref(o), _dptr(yp) if yp is not None else None,
+                       _dptr(sp) if sp is not None else None))
         return o.as_dict(self.S), yp, sp
 
     def save_fields(self, directory: str, it: int):

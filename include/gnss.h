@@ -152,6 +152,15 @@ int gnss_step(gnss_ctx* ctx, int nsteps);
  *   averages per y.  spectrum: (1 + n_scalars) x n_bins doubles, shell sums over
  *   bin = nint(|m|) of 1/2 |u^|^2 (then 1/2 |phi^|^2 per scalar).  Either pointer may be NULL. */
 int gnss_stats(gnss_ctx* ctx, gnss_stats_out* out, double* yprof, double* spectrum);
+/* No counterpart in the reference: gnss_stats for callers that cannot allocate (code under HIP
+ *   graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same outputs, bitwise
+ *   equal to gnss_stats'; the partial sums live in a fixed 40 MiB device scratch per GPU (held until
+ *   the process ends, its address resolved by gnss_create) instead of the context's buffers, and the
+ *   call allocates nothing on the device or the host.  A grid whose partials need more scratch
+ *   (only very anisotropic boxes: the shell count grows with the longest box side over the shortest)
+ *   gets GNSS_ERR_ALLOC.  Every call on one GPU shares its scratch: calls that overlap on one GPU
+ *   (from several threads or contexts) race. */
+int gnss_stats_noalloc(gnss_ctx* ctx, gnss_stats_out* out, double* yprof, double* spectrum);
 
 /* ≈ save3Dfields / importVelocity host copies (iofuncs.cu:329-432, 499-535).
  *   space == GNSS_SPECTRAL: the spectral state; GNSS_PHYSICAL: the field's physical buffer. */

@@ -1335,6 +1335,10 @@ EXPORT int gnss_ensemble_get_field(gnss_ens* e, int i, int f, int s, double* h) 
   }
   return gnss_get_field(e->m[i], f, s, h);
 }
+/* The CPU oracle has no device scratch: the allocation-free statistics call is gnss_stats. */
+EXPORT int gnss_stats_noalloc(gnss_ctx* c, gnss_stats_out* o, double* yp, double* sp) {
+  return gnss_stats(c, o, yp, sp);
+}
 EXPORT int gnss_ensemble_autotune(gnss_ens* e, int r, char* b, int n) {
   (void)r; (void)b; (void)n;
   if (!e) return GNSS_ERR_ARG;
