@@ -1789,12 +1789,14 @@ class Engine : public EngineBase {
     for (int f = 0; f < F_; ++f) a.nl[f] = nl_[f];
     a.nrows = (long long)bm_ * nx_ * ny_;  // members lie back to back: their rows continue
     a.kzs = kzs_;
-    a.lim = kz_lim_;
     a.S = S_;
     a.colloid = colloid_;
     a.alpha = (T)p_.alpha;
     a.late = ((p_.flags & GNSS_FLAG_UNFUSED) == 0 && x_mode() >= 4) ? 1 : 0;
     a.kz = tkz_;
+    if (kz_lim_ != zfused_lim<T>(nz_))
+      throw Error(GNSS_ERR_UNSUPPORTED, "fused z pass: pruned kz limit " + std::to_string(kz_lim_) + " differs from " +
+                                            std::to_string(zfused_lim<T>(nz_)));
     const T scale = (T)(1.0 / ((double)nx_ * ny_ * nz_));
     const double bytes = (double)bm_ * nx_ * ny_ * (double)kz_lim_ * sizeof(C) * (W_ + F_);
     timed("fft_z_fused", bytes, [&] { GNSS_Z_SWITCH(zfused_launch, a, scale); });

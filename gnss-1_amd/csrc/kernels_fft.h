@@ -521,11 +521,17 @@ template <typename T, int M> struct FusedPlan<T, M, 8> : FusedPlan<T, M, 7> {
 };
 
 
+// kz entries a substage's rows hold: kz <= nz/3 (the 2/3 rule), rounded up to the widest column tile
+// of the strided passes (16 complex in FP32, 8 in FP64); the engine's pruned kz limit.
+template <typename T> __host__ __device__ constexpr int zfused_lim(int nz) {
+  return (nz / 3 + 1 + (sizeof(T) == 8 ? 7 : 15)) / (sizeof(T) == 8 ? 8 : 16) * (sizeof(T) == 8 ? 8 : 16);
+}
+
 template <typename T> struct ZFusedArgs {
   const cx<T>* ws[MAXW];
   cx<T>* nl[MAXF];       // (u x omega)_x,y,z, then the advection term of each scalar
   long long nrows;
-  int kzs, lim, S, G;
+  int kzs, S, G;
   int colloid;           // 1: scalar S - 1 is a colloid drifting with alpha grad(scalar 0)
   T alpha;
   // late-derivative mode: the rows hold u, v, w, w_y, w_x, omega_z and per scalar phi_x, phi_y, phi;
@@ -569,6 +575,10 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   // of 1/2, no extra instruction).
   constexpr bool DEFER = FP::defer;
   const T h = DEFER ? (T)0.5 * scale * scale : (T)0.5;
+  // the pruned row length is the same for every launch of a grid (kz below the 2/3 rule, rounded up
+  // to the widest column tile): as a constant it lets the compiler drop the loads, shuffles and
+  // packing of the entries known to be zero and the unpacking of the entries never stored
+  constexpr int LIM = zfused_lim<T>(NZ);
   const long long row = (long long)blockIdx.x * a.G + g;
   const bool active = row < a.nrows;
   const long long off = (active ? row : 0) * a.kzs;
@@ -576,30 +586,30 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   typename Z::Raw r0[3], r1[3];
   cx<T> vel[3][E], om[3][E];
 #pragma unroll
-  for (int b = 0; b < 3; ++b) Z::load(a.ws[b] + off, active, a.lim, t, r0[b]);
+  for (int b = 0; b < 3; ++b) Z::load(a.ws[b] + off, active, LIM, t, r0[b]);
   // late mode: kz of this thread's positions, read once per row
   T kzr[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) kzr[e] = (a.late && t + e * Z::TPC < a.lim) ? a.kz[t + e * Z::TPC] : (T)0;
+  for (int e = 0; e < E; ++e) kzr[e] = (a.late && t + e * Z::TPC < LIM) ? a.kz[t + e * Z::TPC] : (T)0;
   auto load_omega = [&]() {
     if (a.late) {
-      Z::template load_ikz<1, -1>(a.ws[3] + off, a.ws[1] + off, kzr, a.kz, active, a.lim, t, r1[0]);  // w_y - i kz v
-      Z::template load_ikz<-1, 1>(a.ws[4] + off, a.ws[0] + off, kzr, a.kz, active, a.lim, t, r1[1]);  // i kz u - w_x
-      Z::load(a.ws[5] + off, active, a.lim, t, r1[2]);
+      Z::template load_ikz<1, -1>(a.ws[3] + off, a.ws[1] + off, kzr, a.kz, active, LIM, t, r1[0]);  // w_y - i kz v
+      Z::template load_ikz<-1, 1>(a.ws[4] + off, a.ws[0] + off, kzr, a.kz, active, LIM, t, r1[1]);  // i kz u - w_x
+      Z::load(a.ws[5] + off, active, LIM, t, r1[2]);
     } else {
 #pragma unroll
-      for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, a.lim, t, r1[b]);
+      for (int b = 0; b < 3; ++b) Z::load(a.ws[3 + b] + off, active, LIM, t, r1[b]);
     }
   };
   auto load_grad = [&](int s) {
     const int w0 = 6 + 3 * s;
     if (a.late) {
-      Z::load(a.ws[w0] + off, active, a.lim, t, r0[0]);
-      Z::load(a.ws[w0 + 1] + off, active, a.lim, t, r0[1]);
-      Z::template load_ikz<0, 1>(a.ws[w0 + 2] + off, a.ws[w0 + 2] + off, kzr, a.kz, active, a.lim, t, r0[2]);
+      Z::load(a.ws[w0] + off, active, LIM, t, r0[0]);
+      Z::load(a.ws[w0 + 1] + off, active, LIM, t, r0[1]);
+      Z::template load_ikz<0, 1>(a.ws[w0 + 2] + off, a.ws[w0 + 2] + off, kzr, a.kz, active, LIM, t, r0[2]);
     } else {
 #pragma unroll
-      for (int b = 0; b < 3; ++b) Z::load(a.ws[w0 + b] + off, active, a.lim, t, r0[b]);
+      for (int b = 0; b < 3; ++b) Z::load(a.ws[w0 + b] + off, active, LIM, t, r0[b]);
     }
   };
   if (PREFETCH) load_omega();
@@ -622,7 +632,7 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   }
   {
     cx<T>* dst[3] = {a.nl[0] + off, a.nl[1] + off, a.nl[2] + off};
-    Z::template r2c<3>(om, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN, h);
+    Z::template r2c<3>(om, dst, active, LIM, a.kzs, lds, idx, t, twM, twN, h);
   }
   for (int j = 0; j < P; ++j) {
     const int s = a.colloid ? (j + 1 < P ? j + 1 : 0) : j;
@@ -642,14 +652,14 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
         for (int e = 0; e < E; ++e) fma_pair(vel[b][e], mk<T>(a.alpha, a.alpha), om[b][e]);
     }
     cx<T>* dst[1] = {a.nl[3 + s] + off};
-    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN, h);
+    Z::template r2c<1>(n, dst, active, LIM, a.kzs, lds, idx, t, twM, twN, h);
   }
   if (a.colloid) {
     // colloidAdvectionKernel_mgpu (solver.cu:374-391): u.grad c + alpha (lap Z c + grad Z . grad c)
     // = (u + alpha grad Z) . grad c + alpha lap Z c, with u + alpha grad Z already in vel.
     const int sc = a.S - 1;
 #pragma unroll
-    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * sc + b] + off, active, a.lim, t, r0[b]);
+    for (int b = 0; b < 3; ++b) Z::load(a.ws[6 + 3 * sc + b] + off, active, LIM, t, r0[b]);
     Z::template c2r<3, !DEFER>(r0, lds, idx, t, twM, twN, scale, om);  // grad c
     cx<T> n[1][E];
 #pragma unroll
@@ -659,12 +669,12 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
                       u.y * om[0][e].y + v.y * om[1][e].y + w.y * om[2][e].y);
     }
 #pragma unroll
-    for (int b = 0; b < 2; ++b) Z::load(a.ws[6 + 3 * a.S + b] + off, active, a.lim, t, r0[b]);
+    for (int b = 0; b < 2; ++b) Z::load(a.ws[6 + 3 * a.S + b] + off, active, LIM, t, r0[b]);
     Z::template c2r<2, !DEFER>(r0, lds, idx, t, twM, twN, scale, om);  // c, lap Z
 #pragma unroll
     for (int e = 0; e < E; ++e) fma_pair(n[0][e], mk<T>(a.alpha * om[1][e].x, a.alpha * om[1][e].y), om[0][e]);
     cx<T>* dst[1] = {a.nl[3 + sc] + off};
-    Z::template r2c<1>(n, dst, active, a.lim, a.kzs, lds, idx, t, twM, twN, h);
+    Z::template r2c<1>(n, dst, active, LIM, a.kzs, lds, idx, t, twM, twN, h);
   }
 }
 
