@@ -556,7 +556,7 @@ class Engine : public EngineBase {
     if ((q = strstr(p, "yc=")) || (q = strstr(p, "\"yc\": "))) yc = atoi(strchr(q, q[2] == '=' ? '=' : ':') + 1);
     if (!chunk_ok(nx_, cx) || !chunk_ok(ny_, cy) || !zvariant_ok(zf) || xf < 0 || xf > max_xmode() ||
         cp < 0 || cp > 1 || !xchunk_ok(xc) || zp < 0 || zp > 1 || kp < 0 || kp > 4 || cb < 0 || cb > 1 || lp < 0 ||
-        lp > 2 || (yc != 0 && !ychunk_ok(yc)))
+        lp > 3 || (yc != 0 && !ychunk_ok(yc)))
       throw Error(GNSS_ERR_ARG, "invalid tuning '" + spec + "' for " + tuning_key());
     if (kp != kp_ && nsp_ > 0)
       throw Error(GNSS_ERR_ARG, "kp (row padding) is fixed when the context is created: set it in the tuning table");
@@ -748,7 +748,7 @@ class Engine : public EngineBase {
           for (int v = 0; v <= max_xmode(); ++v) {
             for (int c : {4, 8, 16}) {
               if (v == 0 ? c != 4 : !xchunk_ok(c)) continue;  // xf = 0 is timed once per cb
-              for (int l = (v >= 4 ? 0 : 1); l <= (v >= 4 ? 2 : 1); ++l) {  // lp: late modes only
+              for (int l = (v >= 4 ? 0 : 1); l <= (v >= 4 ? 3 : 1); ++l) {  // lp: late modes only
                 cb_ = b;
                 xf_ = v;
                 lp_ = l;
@@ -867,7 +867,7 @@ class Engine : public EngineBase {
           if (!xchunk_ok(c)) continue;
           for (int cp = 0; cp <= 1; ++cp)
             for (int cb = 0; cb <= 1; ++cb)
-              for (int l = 0; l <= 2; ++l) {
+              for (int l = 0; l <= 3; ++l) {
                 xc_ = c; cp_ = cp; cb_ = cb; lp_ = l;
                 const double ms = time_it([&] { substage(1); });
                 snprintf(buf, sizeof buf, "xc=%d cp=%d cb=%d lp=%d", c, cp, cb, l);
@@ -1433,7 +1433,14 @@ class Engine : public EngineBase {
   // prologue and the RK3 update per field group; 2 per-field x inverse passes that form their
   // work field on the fly (forward side as in mode 0); 3 the inverse side of mode 2 with the
   // forward side of mode 1.  The MFMA radix-16 path uses mode 0.
-  int x_mode() const { return (p_.flags & GNSS_FLAG_MFMA_R16) ? 0 : (nb_ > 1 ? 4 : xf_); }
+  int x_mode() const {
+    if (p_.flags & GNSS_FLAG_MFMA_R16) return 0;
+    if (nb_ > 1) return 4;
+    return lane32() ? xf_ : (xf_ == 0 ? 0 : 2);
+  }
+  // The late passes and the fused forward x pass address a tile with a 32-bit byte offset per lane:
+  // its column plus its row t (at most 64 rows per column), rows up to ny kzs elements apart.
+  bool lane32() const { return (64LL * ny_ * kzs_ + 64) * (long long)sizeof(C) <= 0xFFFFFFFFLL; }
   // modes 4 and 5: late derivatives (the x pass reads each state field once, the y and z passes
   // apply their own derivatives; see late_inverse), with mode 2's and mode 3's forward side.
   // Not built for the colloid, whose work fields need second derivatives.
@@ -1499,7 +1506,7 @@ class Engine : public EngineBase {
   // phi_y and omega_z = v_x - i k_y u; the fused z pass forms omega_x = w_y - i k_z v,
   // omega_y = i k_z u - w_x and phi_z = i k_z phi as it loads.
   void late_inverse() {
-    if (nsp_ * bm_ >= (1LL << 31)) throw Error(GNSS_ERR_UNSUPPORTED, "late passes index fields with 32-bit offsets");
+    if (!lane32()) throw Error(GNSS_ERR_UNSUPPORTED, "late passes address a tile with 32-bit byte offsets");
     const PairOp copy{1, 1, {1, 0}, {0, 0}, {0, 0}, {0, 0}};        // out0 = A
     const PairOp copy_d{1, 2, {1, 0}, {0, 1}, {0, 0}, {0, 0}};      // out0 = A, out1 = i k A
     const PairOp curl_z{2, 2, {1, 0}, {0, -1}, {0, 1}, {0, 0}};     // out0 = A, out1 = B - i k A
@@ -1560,7 +1567,8 @@ class Engine : public EngineBase {
     a.mstride = nsp_;
   }
   // With lp = 1 an entry's two outputs go through one launch (lp = 2: transformed one after the
-  // other through one LDS region, which halves the kernel's LDS); with lp = 0 they are split into
+  // other through one LDS region, which halves the kernel's LDS; lp = 3: together, with the next
+  // tile's input loaded before they transform); with lp = 0 they are split into
   // single-output entries that read the inputs again, the second outputs first (they read fields
   // the first outputs overwrite in place).  Then one launch per (inputs, outputs) kind with cb,
   // else one per entry.
@@ -1640,31 +1648,40 @@ class Engine : public EngineBase {
       else pair_k<L, 8>(li, n, a, kv, kind);
     }
   }
+  // two-output form of lp = 1, 2, 3
+  template <int L, int CC, int NIN>
+  void pair_k2(const PairList<T>& li, int n, const ColArgs& a, const T* kv) {
+    if (lp_ == 2) pair_launch<L, CC, NIN, 2, PAIR_SEQ>(li, n, a, kv);
+    else if (lp_ == 3) pair_launch<L, CC, NIN, 2, PAIR_BATCHED_PF>(li, n, a, kv);
+    else pair_launch<L, CC, NIN, 2, PAIR_BATCHED>(li, n, a, kv);
+  }
   template <int L, int CC>
   void pair_k(const PairList<T>& li, int n, const ColArgs& a, const T* kv, int kind) {
-    if (kind == 0) pair_launch<L, CC, 1, 1, false>(li, n, a, kv);
-    else if (kind == 1 && lp_ == 2) pair_launch<L, CC, 1, 2, true>(li, n, a, kv);
-    else if (kind == 1) pair_launch<L, CC, 1, 2, false>(li, n, a, kv);
-    else if (kind == 2 && lp_ == 2) pair_launch<L, CC, 2, 2, true>(li, n, a, kv);
-    else if (kind == 2) pair_launch<L, CC, 2, 2, false>(li, n, a, kv);
-    else pair_launch<L, CC, 2, 1, false>(li, n, a, kv);
+    if (kind == 0) pair_launch<L, CC, 1, 1, PAIR_BATCHED>(li, n, a, kv);
+    else if (kind == 1) pair_k2<L, CC, 1>(li, n, a, kv);
+    else if (kind == 2) pair_k2<L, CC, 2>(li, n, a, kv);
+    else pair_launch<L, CC, 2, 1, PAIR_BATCHED>(li, n, a, kv);
   }
-  template <int L, int CC, int NIN, int NOUT, bool SEQ>
+  template <int L, int CC, int NIN, int NOUT, int FORM>
   void pair_launch(const PairList<T>& li, int n, ColArgs a, const T* kv) {
     constexpr int TPC = L / Plan<L>::E;
-    // two batched outputs use two LDS regions: up to the 160 KiB a gfx950 workgroup can hold
+    // two batched outputs use two LDS regions, and a two-output kernel keeps the twiddles in LDS:
+    // up to the 160 KiB a gfx950 workgroup can hold
     constexpr long long PAIR_LDS_MAX = 160 * 1024;
-    const long long region = (long long)(SEQ ? 1 : NOUT) * L * CC * (long long)sizeof(C);
+    const long long region = (long long)(NOUT == 2 && FORM == PAIR_SEQ ? 1 : NOUT) * L * CC * (long long)sizeof(C);
+    const long long twb = NOUT == 2 ? (long long)L * (long long)sizeof(C) : 0;
     int G = std::max(1, 256 / (CC * TPC));
-    while (G > 1 && G * region > PAIR_LDS_MAX) --G;
-    if (region > PAIR_LDS_MAX) throw Error(GNSS_ERR_UNSUPPORTED, "late pass tile does not fit the LDS");
+    while (G > 1 && G * region + twb > PAIR_LDS_MAX) --G;
+    if (region + twb > PAIR_LDS_MAX) throw Error(GNSS_ERR_UNSUPPORTED, "late pass tile does not fit the LDS");
+    // the kernel takes the rows beyond the 2/3 rule's L / 3 for zeros without reading them
+    if (a.in_keep != L / 3) throw Error(GNSS_ERR_STATE, "late pass input rows must be the kept ones");
     a.G = G;
     const int threads = CC * TPC * G;
-    const size_t lds = (size_t)(G * region);
+    const size_t lds = (size_t)(G * region + twb);
     const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
     const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds, 2048 / threads));
     const long long grid = std::min<long long>(blocks * n, per_cu * n_cu_);
-    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC, NIN, NOUT, SEQ>), dim3((unsigned)grid), dim3(threads), lds, s_, li,
+    hipLaunchKernelGGL((k_inv_pair_pipe<T, L, CC, NIN, NOUT, FORM>), dim3((unsigned)grid), dim3(threads), lds, s_, li,
                        kv, tw_.at(L), a, (int)blocks, (int)(blocks * n));
   }
 
@@ -1724,7 +1741,8 @@ class Engine : public EngineBase {
     constexpr int TPC = L / Plan<L>::E;
     const int nouter = 2 * g_.keep_y + 1;
     const long long blocks = (long long)nchunks * cdiv(nouter, G);
-    const size_t lds = (size_t)G * L * CC * sizeof(C);
+    // the forward kernel keeps a copy of the twiddles after its data
+    const size_t lds = (size_t)G * L * CC * sizeof(C) + (pro ? 0 : (size_t)L * sizeof(C));
     if (pro)
       hipLaunchKernelGGL((k_xinv_prologue<T, L, CC, GRP>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_,
                          g_, ptrs(q_), ptrs(ws_), tkx_, tky_, tkz_, tw_.at(L), scalar, nchunks, G);
@@ -1753,7 +1771,8 @@ class Engine : public EngineBase {
     const int L = nx_, CC = xc_;
     const int TPC = L / elems_per_thread(L);
     int G = std::max(1, 256 / (CC * TPC));
-    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) > LDS_MAX) --G;
+    const long long twb = pro ? 0 : (long long)L * (long long)sizeof(C);
+    while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) + twb > LDS_MAX) --G;
     const int nchunks = kz_lim_ / CC;
     const int nouter = 2 * g_.keep_y + 1;
     const double kept_rows = 2 * g_.keep_x + 1;

@@ -65,6 +65,15 @@ template <typename V> __device__ __forceinline__ V ld_s(const V* p) {
   return *p;
 #endif
 }
+// The element at a (wave-uniform) base plus a 32-bit per-lane byte offset: addressed with the base
+// in scalar registers and one vector register per lane for any number of bases, where a 64-bit
+// element index costs a vector register pair per access.
+template <typename V> __device__ __forceinline__ V ld_at(const V* base, unsigned byte_off) {
+  return ld_s(reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off));
+}
+template <typename V> __device__ __forceinline__ void st_at(V* base, unsigned byte_off, const V& v) {
+  st_s(reinterpret_cast<V*>(reinterpret_cast<char*>(base) + byte_off), v);
+}
 
 // multiply by (SIGN * i)
 template <int SIGN, typename V> __device__ __forceinline__ V cmul_si(V a) {

@@ -175,17 +175,23 @@ __global__ void k_rk(Geom g, Box b, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const 
     }
   }
   const T a = (T)c.a, bb = (T)c.b, dt = (T)c.dt;
+  // every field's loads before the first store (which the compiler cannot move them past)
+  cx<T> oq[MAXF], od[MAXF];
+#pragma unroll
+  for (int f = 0; f < MAXF; ++f) {
+    od[f] = (f < F && !c.first) ? ld_s(&dq.p[f][o]) : mk<T>(0, 0);
+    oq[f] = f < F ? ld_s(&q.p[f][o]) : mk<T>(0, 0);
+  }
 #pragma unroll
   for (int f = 0; f < MAXF; ++f) {
     if (f >= F) break;
     const T eps = -dexpm1<T>(-(T)(c.nu_m ? c.nu_m[m * MAXF + f] : c.nu[f]) * k2 * (T)c.tau);
     cx<T> d = cscale(rhs[f], dt);
     if (!c.first) {
-      const cx<T> old = ld_s(&dq.p[f][o]);
-      d.x = a * old.x + d.x;
-      d.y = a * old.y + d.y;
+      d.x = a * od[f].x + d.x;
+      d.y = a * od[f].y + d.y;
     }
-    cx<T> qq = ld_s(&q.p[f][o]);
+    cx<T> qq = oq[f];
     qq.x = if_update(qq.x, bb, d.x, eps);
     qq.y = if_update(qq.y, bb, d.y, eps);
     st_s(&q.p[f][o], qq);
@@ -651,15 +657,36 @@ template <typename T> struct PairList {
   PairOp op[MAXW];
 };
 
+// Bit e set: every row p of slab e (p in [e TPC, (e + 1) TPC), the rows a thread's element e takes
+// in a length-L transform of TPC threads per column) has |mode| > L / 3, a row the 2/3 rule removes.
+constexpr unsigned removed_slabs(int L, int TPC, int E) {
+  unsigned m = 0;
+  for (int e = 0; e < E; ++e) {
+    bool all = true;
+    for (int p = e * TPC; p < (e + 1) * TPC && all; ++p) {
+      const int md = p < (L >> 1) ? p : p - L;
+      all = (md < 0 ? -md : md) > L / 3;
+    }
+    if (all) m |= 1u << e;
+  }
+  return m;
+}
+
+// Forms of the two-output kernels: both outputs through the radix stages together in two LDS
+// regions, A loaded with its tile (PAIR_BATCHED); one after the other through one region, the
+// next tile's A loaded once the last output is formed (PAIR_SEQ: half the LDS); together, with the
+// next tile's A loaded before they transform (PAIR_BATCHED_PF).
+constexpr int PAIR_BATCHED = 0, PAIR_SEQ = 1, PAIR_BATCHED_PF = 2;
+
 // Persistent like k_xinv_op_pipe: tile i is tile i % tpf of list entry i / tpf; B is loaded when
-// the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch.  Two outputs go
-// through the radix stages together, in two LDS regions (SEQ = false), or one after the other
-// through one region (SEQ = true: half the LDS, and the next tile's A is loaded once the last
-// output is formed, before it transforms).  Element offsets are 32-bit: a field (all members of a
-// batched context together) holds fewer than 2^31 elements (checked by the host).  An entry may write its outputs over its own inputs (each
-// tile is read before it is written, by the one block that owns it, and both outputs are formed
-// from registers); no entry may read a field another entry writes.
-template <typename T, int L, int C, int NIN, int NOUT, bool SEQ>
+// the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch; FORM: how two
+// outputs share the block (one output: always with the next tile's A loaded before the transform).
+// The twiddles of a two-output kernel are read from a copy in LDS, after the data regions: read
+// from memory, each stage would wait for every load in flight, the next tile's included.  An entry
+// may write its outputs over its own inputs (each tile is read before it is written, by the one
+// block that owns it, and both outputs are formed from registers); no entry may read a field
+// another entry writes.
+template <typename T, int L, int C, int NIN, int NOUT, int FORM>
 __global__ void __launch_bounds__(lb_threads(C * (L / Plan<L>::E)))
 k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restrict__ tw, ColArgs a, int tpf,
                 int ntiles) {
@@ -671,31 +698,39 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
   const int c = tid % C;
   const int t = (tid / C) % TPC;
   const int g = tid / (C * TPC);
-  auto offset_of = [&](int tile, bool& active) {
+  // Element (row p = t + e TPC, column c) of a tile sits at tile_off(tile) + e TPC estride (the
+  // same for the whole group: with one group per block, wave-uniform) plus the lane's
+  // c + t estride, which the host keeps below 2^32 bytes.
+  auto tile_off = [&](int tile, bool& active) {
     const int lt = tile % tpf;
     const int chunk = lt % a.nchunks;
     const int o = (lt / a.nchunks) * a.G + g;
     active = o < a.nouter;
     int om = active ? o : 0;
-    const unsigned mo = (unsigned)member_split(om, a.nouter_m, a.mstride);
+    const long long mo = member_split(om, a.nouter_m, a.mstride);
     const int io = outer_index(om, a.outer_keep, a.outer_n);
-    return mo + (unsigned)io * (unsigned)a.ostride + (unsigned)(chunk * C + c);
+    return mo + (long long)io * a.ostride + chunk * C;
   };
+  const unsigned lane_b = (unsigned)(c + (long long)t * a.estride) * (unsigned)sizeof(cx<T>);
+  // the inputs hold the rows the 2/3 rule keeps (in_keep = L / 3, checked by the host): the slabs
+  // of removed rows are zeros known at compile time, neither loaded nor transformed
+  constexpr unsigned ZS = removed_slabs(L, TPC, E);
   auto load_col = [&](const cx<T>* __restrict__ src, int tile, cx<T>* va) {
     bool active;
-    const unsigned off = offset_of(tile, active);
+    const long long off = tile_off(tile, active);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int p = t + e * TPC;
-      va[e] = (active && row_kept(p, L, a.in_keep)) ? ld_s(&src[off + (unsigned)p * (unsigned)a.estride]) : mk<T>(0, 0);
+      va[e] = (!((ZS >> e) & 1u) && active && row_kept(p, L, a.in_keep))
+                  ? ld_at(src + off + (long long)e * TPC * a.estride, lane_b)
+                  : mk<T>(0, 0);
     }
   };
+  constexpr bool SEQ = NOUT == 2 && FORM == PAIR_SEQ;
   // G groups of C columns; each group owns NREG regions of L x C entries
   constexpr int NREG = SEQ ? 1 : NOUT;
   const LdsCol idx{g * NREG * L * C + c, C};
-  // the next tile's A is loaded before the current tile transforms with one output or SEQ; with
-  // two outputs batched A is loaded with the tile (three sets of columns would not fit the registers)
-  constexpr bool PF = NOUT == 1 || SEQ;
+  constexpr bool PF = NOUT == 1 || FORM != PAIR_BATCHED;
   // output j of entry op at this thread's positions
   auto form = [&](const PairOp& op, int j, const cx<T>* va, const cx<T>* vb, cx<T>* r) {
 #pragma unroll
@@ -715,6 +750,13 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
   int tile = blockIdx.x;
   cx<T> va[E];
   if (PF) load_col(li.A[tile / tpf], tile, va);
+  const cx<T>* twl = tw;
+  if constexpr (NOUT == 2) {
+    cx<T>* tws = lds + (long long)a.G * NREG * L * C;
+    for (int i = tid; i < L; i += blockDim.x) tws[i] = tw[i];
+    __syncthreads();
+    twl = tws;
+  }
   while (true) {
     const int ent = tile / tpf;
     const PairOp op = li.op[ent];
@@ -723,12 +765,8 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     if (NIN == 2) load_col(li.B[ent], tile, vb);
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
-    // with two outputs the twiddles are re-read for each tile rather than held in registers
-    // across the tile loop (the registers are needed for the second output)
-    const cx<T>* twl = tw;
-    if (NOUT == 2) asm volatile("" : "+s"(twl));
     bool active;
-    const unsigned off = offset_of(tile, active);
+    const long long off = tile_off(tile, active);
     if constexpr (SEQ) {
 #pragma unroll
       for (int j = 0; j < NOUT; ++j) {
@@ -739,7 +777,7 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
         if (active) {
           cx<T>* __restrict__ dst = li.out[j][ent];
 #pragma unroll
-          for (int e = 0; e < E; ++e) st_s(&dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride], v[0][e]);
+          for (int e = 0; e < E; ++e) st_at(dst + off + (long long)e * TPC * a.estride, lane_b, v[0][e]);
         }
       }
     } else {
@@ -753,7 +791,7 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
         for (int j = 0; j < NOUT; ++j) {
           cx<T>* __restrict__ dst = li.out[j][ent];
 #pragma unroll
-          for (int e = 0; e < E; ++e) st_s(&dst[off + (unsigned)(t + e * TPC) * (unsigned)a.estride], v[j][e]);
+          for (int e = 0; e < E; ++e) st_at(dst + off + (long long)e * TPC * a.estride, lane_b, v[j][e]);
         }
       }
     }
@@ -786,70 +824,106 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   const bool active = o < nouter;
   const int iy = outer_index(active ? o : 0, g.keep_y, g.ny);
   const int iz = chunk * C + c;
-  const long long col = (long long)iy * g.kzs + iz;
   const long long estride = (long long)g.ny * g.kzs;
+  // element (row p = t + e TPC, column c) at at(e) (wave-uniform with one group per block) plus the
+  // lane's c + t estride, which the host keeps below 2^32 bytes
+  const long long base = (long long)iy * g.kzs + chunk * C;
+  auto at = [&](int e) { return base + (long long)e * TPC * estride; };
+  const unsigned lane_b = (unsigned)(c + (long long)t * estride) * (unsigned)sizeof(cx<T>);
   const LdsCol idx{gg * L * C + c, C};
   const int f0 = VEL ? 0 : 3 + scalar;
+  // every field's loads are issued before the first transform (the registers hold all NF fields
+  // after the transforms anyway), so the later fields arrive while the earlier ones transform; the
+  // twiddles are read from a copy in LDS after the G L C data entries (read from memory, each radix
+  // stage would wait for all of those loads)
   cx<T> nv[NF][E];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
+  for (int f = 0; f < NF; ++f)
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      nv[f][e] = active ? ld_s(&nl.p[f0 + f][col + (long long)(t + e * TPC) * estride]) : mk<T>(0, 0);
+    for (int e = 0; e < E; ++e) nv[f][e] = active ? ld_at(nl.p[f0 + f] + at(e), lane_b) : mk<T>(0, 0);
+  cx<T>* tws = lds + (long long)G * L * C;
+  for (int i = tid; i < L; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
     if (f > 0) __syncthreads();
-    fft<T, L, -1>(nv[f], lds, t, idx, tw);
+    fft<T, L, -1>(nv[f], lds, t, idx, (const cx<T>*)tws);
   }
   if (!active) return;
   const T ky = kyv[iy], kz = kzv[iz];
   const int my = mode_of(iy, g.ny), mz = mode_of(iz, g.nz);
   const T a = (T)rc.a, bb = (T)rc.b, dt = (T)rc.dt;
   const T nu = (T)rc.nu[f0];
+  // The state and RK-register values of EB elements are loaded before any of them is stored: the
+  // stores may alias the loads as far as the compiler knows, so it keeps them in program order,
+  // and loading per element and field would cost one HBM round trip each.
+  // With three transformed fields in registers the velocity group has no room to load ahead (its
+  // loads go field by field); a scalar loads up to four elements at a time.
+  constexpr bool PRE = !VEL;
+  constexpr int EB = PRE ? (E / 2 < 4 ? E / 2 : 4) : 1;
+  static_assert(E % EB == 0, "element batches must divide the elements of a thread");
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int p = t + e * TPC;
-    if (!row_kept(p, L, g.keep_x)) continue;
-    const long long o2 = col + (long long)p * estride;
-    if (iz >= g.nzh || !kept(g, mode_of(p, g.nx), my, mz)) {
+  for (int e0 = 0; e0 < E; e0 += EB) {
+    cx<T> oq[EB][NF], od[EB][NF];
+#pragma unroll
+    for (int j = 0; j < EB && PRE; ++j) {
+      const int p = t + (e0 + j) * TPC;
+      const long long o2 = at(e0 + j);
+      const bool live = row_kept(p, L, g.keep_x) && iz < g.nzh && kept(g, mode_of(p, g.nx), my, mz);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        st_s(&q.p[f0 + f][o2], mk<T>(0, 0));
-        if (!rc.last) st_s(&dq.p[f0 + f][o2], mk<T>(0, 0));
+        od[j][f] = (live && !rc.first) ? ld_at(dq.p[f0 + f] + o2, lane_b) : mk<T>(0, 0);
+        oq[j][f] = live ? ld_at(q.p[f0 + f] + o2, lane_b) : mk<T>(0, 0);
       }
-      continue;
     }
-    const T kx = kxv[p];
-    const T k2 = kx * kx + ky * ky + kz * kz;
-    cx<T> rhs[NF];
-    if (k2 == (T)0) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) rhs[f] = mk<T>(0, 0);
-    } else if (VEL) {
-      const cx<T> n0 = nv[0][e], n1 = nv[NF > 1 ? 1 : 0][e], n2 = nv[NF - 1][e];
-      cx<T> kn;
-      kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
-      kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
-      rhs[0] = csub(n0, cscale(kn, kx));
-      if (NF > 1) {
-        rhs[NF > 1 ? 1 : 0] = csub(n1, cscale(kn, ky));
-        rhs[NF - 1] = csub(n2, cscale(kn, kz));
-      }
-    } else {
-      rhs[0] = mk<T>(-nv[0][e].x, -nv[0][e].y);
-    }
-    const T eps = -dexpm1<T>(-nu * k2 * (T)rc.tau);
+    for (int j = 0; j < EB; ++j) {
+      const int e = e0 + j;
+      const int p = t + e * TPC;
+      if (!row_kept(p, L, g.keep_x)) continue;
+      const long long o2 = at(e);
+      if (iz >= g.nzh || !kept(g, mode_of(p, g.nx), my, mz)) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      cx<T> d = cscale(rhs[f], dt);
-      if (!rc.first) {
-        const cx<T> old = ld_s(&dq.p[f0 + f][o2]);
-        d.x = a * old.x + d.x;
-        d.y = a * old.y + d.y;
+        for (int f = 0; f < NF; ++f) {
+          st_at(q.p[f0 + f] + o2, lane_b, mk<T>(0, 0));
+          if (!rc.last) st_at(dq.p[f0 + f] + o2, lane_b, mk<T>(0, 0));
+        }
+        continue;
       }
-      cx<T> qq = ld_s(&q.p[f0 + f][o2]);
-      qq.x = if_update(qq.x, bb, d.x, eps);
-      qq.y = if_update(qq.y, bb, d.y, eps);
-      st_s(&q.p[f0 + f][o2], qq);
-      if (!rc.last) st_s(&dq.p[f0 + f][o2], mk<T>(apply_if(d.x, eps), apply_if(d.y, eps)));
+      const T kx = kxv[p];
+      const T k2 = kx * kx + ky * ky + kz * kz;
+      cx<T> rhs[NF];
+      if (k2 == (T)0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) rhs[f] = mk<T>(0, 0);
+      } else if (VEL) {
+        const cx<T> n0 = nv[0][e], n1 = nv[NF > 1 ? 1 : 0][e], n2 = nv[NF - 1][e];
+        cx<T> kn;
+        kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
+        kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
+        rhs[0] = csub(n0, cscale(kn, kx));
+        if (NF > 1) {
+          rhs[NF > 1 ? 1 : 0] = csub(n1, cscale(kn, ky));
+          rhs[NF - 1] = csub(n2, cscale(kn, kz));
+        }
+      } else {
+        rhs[0] = mk<T>(-nv[0][e].x, -nv[0][e].y);
+      }
+      const T eps = -dexpm1<T>(-nu * k2 * (T)rc.tau);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        cx<T> d = cscale(rhs[f], dt);
+        if (!rc.first) {
+          const cx<T> old = PRE ? od[j][f] : ld_at(dq.p[f0 + f] + o2, lane_b);
+          d.x = a * old.x + d.x;
+          d.y = a * old.y + d.y;
+        }
+        cx<T> qq = PRE ? oq[j][f] : ld_at(q.p[f0 + f] + o2, lane_b);
+        qq.x = if_update(qq.x, bb, d.x, eps);
+        qq.y = if_update(qq.y, bb, d.y, eps);
+        st_at(q.p[f0 + f] + o2, lane_b, qq);
+        if (!rc.last) st_at(dq.p[f0 + f] + o2, lane_b, mk<T>(apply_if(d.x, eps), apply_if(d.y, eps)));
+      }
     }
   }
 }

@@ -198,8 +198,8 @@ def test_late_derivative_modes_match(g, engine, prec, n, sc):
     omega_x, omega_y and phi_z with i k_z as it loads).  Same step as modes 2 and 3, which form
     every work field from the state in the x pass, to rounding (FP64 1e-13, FP32 2e-6); with one
     launch per pass (cb = 1) and per field, with the two outputs of an entry in one launch (lp = 1,
-    together; lp = 2, one after the other) or split, with each fused z variant and with the y
-    passes' own tile (yc)."""
+    together; lp = 2, one after the other; lp = 3, together with the next tile loaded ahead) or
+    split, with each fused z variant and with the y passes' own tile (yc)."""
     cfg = g.Config(n=n, re=400.0, sc=sc, theta=0.08, noise_kmax=8.0, noise_amp=0.2, dt=1e-3, precision=prec)
     nf = 3 + len(sc)
     zfs = (0, 3, 6, 7) if prec == 0 else (0, 1, 3, 6, 7)
@@ -211,7 +211,8 @@ def test_late_derivative_modes_match(g, engine, prec, n, sc):
         want = [ref.get_spectral(f) for f in range(nf)]
         ref.close()
         ycs = ["cb=1 yc=%d" % c for c in ((4, 8) if prec == 0 else (8, 16))]
-        for extra in ["cb=1 zf=%d" % z for z in zfs] + ["cb=0", "cb=1 lp=0", "cb=0 lp=0", "cb=1 lp=2", "cb=0 lp=2"] + ycs:
+        for extra in ["cb=1 zf=%d" % z for z in zfs] + ["cb=0", "cb=1 lp=0", "cb=0 lp=0", "cb=1 lp=2", "cb=0 lp=2",
+                                                        "cb=1 lp=3", "cb=0 lp=3"] + ycs:
             a = g.Solver(cfg, engine).setup()
             a.set_tuning(late + " " + extra)
             a.step(3)
