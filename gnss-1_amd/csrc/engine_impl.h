@@ -1741,8 +1741,13 @@ class Engine : public EngineBase {
     constexpr int TPC = L / Plan<L>::E;
     const int nouter = 2 * g_.keep_y + 1;
     const long long blocks = (long long)nchunks * cdiv(nouter, G);
-    // the forward kernel keeps a copy of the twiddles after its data
-    const size_t lds = (size_t)G * L * CC * sizeof(C) + (pro ? 0 : (size_t)L * sizeof(C));
+    // the forward kernel keeps a copy of the twiddles after its data region, which the velocity
+    // group's right-hand sides may take over once the transforms are done
+    const long long data = (long long)G * L * CC;
+    const long long rhs = xfwd_rhs_entries<T, L, CC, GRP == 0>(CC * TPC * G);
+    const size_t lds = pro ? (size_t)data * sizeof(C) : (size_t)(std::max(data, rhs) + L) * sizeof(C);
+    if (lds > 160 * 1024) throw Error(GNSS_ERR_UNSUPPORTED, "fused x pass tile does not fit the LDS");
+    if (!pro && g_.keep_x != L / 3) throw Error(GNSS_ERR_STATE, "fused x pass rows must be the kept ones");
     if (pro)
       hipLaunchKernelGGL((k_xinv_prologue<T, L, CC, GRP>), dim3((unsigned)blocks), dim3(CC * TPC * G), lds, s_,
                          g_, ptrs(q_), ptrs(ws_), tkx_, tky_, tkz_, tw_.at(L), scalar, nchunks, G);

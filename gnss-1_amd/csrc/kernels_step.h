@@ -800,6 +800,17 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
   }
 }
 
+// The velocity group of the fused forward x pass parks its projected right-hand sides in LDS
+// (one slot per thread, field and element of a slab the 2/3 rule keeps) when they fit beside the
+// twiddles: its registers are then free to load the state of several elements before any store.
+// Entries of that region for blocks of `threads` threads (0: the sides stay in registers).
+template <typename T, int L, int C, bool VEL> __host__ __device__ constexpr long long xfwd_rhs_entries(int threads) {
+  constexpr int E = Plan<L>::E, TPC = L / E;
+  constexpr int NK = E - __builtin_popcount(removed_slabs(L, TPC, E));
+  constexpr long long most = (long long)NK * 3 * lb_threads(C * TPC) * (long long)sizeof(cx<T>) + L * (long long)sizeof(cx<T>);
+  return (VEL && most <= 160 * 1024) ? (long long)NK * 3 * threads : 0;
+}
+
 // Forward x pass of one field group's nonlinear terms fused with the projection (velocity),
 // the sign of the scalar term (computeRHSKernel_mgpu solver.cu:461-521), the mask and the RK3-IF
 // update of the state (replaces solver.cu:547-657): the transformed nonlinear terms never return
@@ -841,7 +852,8 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int e = 0; e < E; ++e) nv[f][e] = active ? ld_at(nl.p[f0 + f] + at(e), lane_b) : mk<T>(0, 0);
-  cx<T>* tws = lds + (long long)G * L * C;
+  const long long rhs_n = xfwd_rhs_entries<T, L, C, VEL>(blockDim.x);
+  cx<T>* tws = lds + ((long long)G * L * C > rhs_n ? (long long)G * L * C : rhs_n);
   for (int i = tid; i < L; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
 #pragma unroll
@@ -854,12 +866,51 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   const int my = mode_of(iy, g.ny), mz = mode_of(iz, g.nz);
   const T a = (T)rc.a, bb = (T)rc.b, dt = (T)rc.dt;
   const T nu = (T)rc.nu[f0];
+  // right-hand side of element e: the projected nonlinear term (velocity) or its negative (scalar)
+  auto rhs_of = [&](int e, cx<T>* rhs) {
+    const T kx = kxv[t + e * TPC];
+    const T k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 == (T)0) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) rhs[f] = mk<T>(0, 0);
+    } else if (VEL) {
+      const cx<T> n0 = nv[0][e], n1 = nv[NF > 1 ? 1 : 0][e], n2 = nv[NF - 1][e];
+      cx<T> kn;
+      kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
+      kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
+      rhs[0] = csub(n0, cscale(kn, kx));
+      if (NF > 1) {
+        rhs[NF > 1 ? 1 : 0] = csub(n1, cscale(kn, ky));
+        rhs[NF - 1] = csub(n2, cscale(kn, kz));
+      }
+    } else {
+      rhs[0] = mk<T>(-nv[0][e].x, -nv[0][e].y);
+    }
+  };
+  // Slabs of rows the 2/3 rule removes are neither read nor written (the host checks keep_x = L / 3).
+  constexpr unsigned ZS = removed_slabs(L, TPC, E);
+  constexpr bool RL = xfwd_rhs_entries<T, L, C, VEL>(1) > 0;
+  // slot of (kept slab e, field f) in the LDS region of the right-hand sides: thread-private, so no
+  // barrier (every thread has read its last exchange of the transforms, which end in one)
+  auto slot = [&](int e, int f) {
+    const int ek = e - __builtin_popcount(ZS & ((1u << e) - 1u));
+    return lds + ((long long)(ek * NF + f) * blockDim.x + tid);
+  };
+  if constexpr (RL) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if ((ZS >> e) & 1u) continue;
+      cx<T> rhs[NF];
+      rhs_of(e, rhs);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) *slot(e, f) = rhs[f];
+    }
+  }
   // The state and RK-register values of EB elements are loaded before any of them is stored: the
   // stores may alias the loads as far as the compiler knows, so it keeps them in program order,
-  // and loading per element and field would cost one HBM round trip each.
-  // With three transformed fields in registers the velocity group has no room to load ahead (its
-  // loads go field by field); a scalar loads up to four elements at a time.
-  constexpr bool PRE = !VEL;
+  // and loading per element and field would cost one HBM round trip each.  A velocity group whose
+  // right-hand sides stay in registers has no room to load ahead (its loads go field by field).
+  constexpr bool PRE = !VEL || RL;
   constexpr int EB = PRE ? (E / 2 < 4 ? E / 2 : 4) : 1;
   static_assert(E % EB == 0, "element batches must divide the elements of a thread");
 #pragma unroll
@@ -869,7 +920,8 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
     for (int j = 0; j < EB && PRE; ++j) {
       const int p = t + (e0 + j) * TPC;
       const long long o2 = at(e0 + j);
-      const bool live = row_kept(p, L, g.keep_x) && iz < g.nzh && kept(g, mode_of(p, g.nx), my, mz);
+      const bool live = !((ZS >> (e0 + j)) & 1u) && row_kept(p, L, g.keep_x) && iz < g.nzh &&
+                        kept(g, mode_of(p, g.nx), my, mz);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         od[j][f] = (live && !rc.first) ? ld_at(dq.p[f0 + f] + o2, lane_b) : mk<T>(0, 0);
@@ -880,7 +932,7 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
     for (int j = 0; j < EB; ++j) {
       const int e = e0 + j;
       const int p = t + e * TPC;
-      if (!row_kept(p, L, g.keep_x)) continue;
+      if (((ZS >> e) & 1u) || !row_kept(p, L, g.keep_x)) continue;
       const long long o2 = at(e);
       if (iz >= g.nzh || !kept(g, mode_of(p, g.nx), my, mz)) {
 #pragma unroll
@@ -893,21 +945,11 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
       const T kx = kxv[p];
       const T k2 = kx * kx + ky * ky + kz * kz;
       cx<T> rhs[NF];
-      if (k2 == (T)0) {
+      if constexpr (RL) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) rhs[f] = mk<T>(0, 0);
-      } else if (VEL) {
-        const cx<T> n0 = nv[0][e], n1 = nv[NF > 1 ? 1 : 0][e], n2 = nv[NF - 1][e];
-        cx<T> kn;
-        kn.x = (kx * n0.x + ky * n1.x + kz * n2.x) / k2;
-        kn.y = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
-        rhs[0] = csub(n0, cscale(kn, kx));
-        if (NF > 1) {
-          rhs[NF > 1 ? 1 : 0] = csub(n1, cscale(kn, ky));
-          rhs[NF - 1] = csub(n2, cscale(kn, kz));
-        }
+        for (int f = 0; f < NF; ++f) rhs[f] = *slot(e, f);
       } else {
-        rhs[0] = mk<T>(-nv[0][e].x, -nv[0][e].y);
+        rhs_of(e, rhs);
       }
       const T eps = -dexpm1<T>(-nu * k2 * (T)rc.tau);
 #pragma unroll
