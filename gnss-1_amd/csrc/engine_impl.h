@@ -1292,8 +1292,9 @@ class Engine : public EngineBase {
     const size_t lds = (size_t)G * zrow_stride(M) * sizeof(C);
     if (zp_) {
       const long long groups = cdiv(rows, G);
-      hipLaunchKernelGGL((k_r2c_z_pipe<T, NZ>), dim3((unsigned)std::min(groups, zpipe_blocks(lds, TPC * G))),
-                         dim3(TPC * G), lds, s_, in, out, tw_.at(M), tw_.at(NZ), rows, kzs_, out_lim, G, groups);
+      const size_t ldsp = lds + 2 * (size_t)M * sizeof(C);  // and both twiddle tables
+      hipLaunchKernelGGL((k_r2c_z_pipe<T, NZ>), dim3((unsigned)std::min(groups, zpipe_blocks(ldsp, TPC * G))),
+                         dim3(TPC * G), ldsp, s_, in, out, tw_.at(M), tw_.at(NZ), rows, kzs_, out_lim, G, groups);
       return;
     }
     hipLaunchKernelGGL((k_r2c_z<T, NZ>), dim3((unsigned)cdiv(rows, G)), dim3(TPC * G), lds, s_, in, out,
@@ -1309,8 +1310,9 @@ class Engine : public EngineBase {
     const size_t lds = (size_t)G * zrow_stride(M) * sizeof(C);
     if (zp_) {
       const long long groups = cdiv(rows, G);
-      hipLaunchKernelGGL((k_c2r_z_pipe<T, NZ>), dim3((unsigned)std::min(groups, zpipe_blocks(lds, TPC * G))),
-                         dim3(TPC * G), lds, s_, in, out, tw_.at(M), tw_.at(NZ), rows, kzs_, in_lim, scale, G,
+      const size_t ldsp = lds + 2 * (size_t)M * sizeof(C);  // and both twiddle tables
+      hipLaunchKernelGGL((k_c2r_z_pipe<T, NZ>), dim3((unsigned)std::min(groups, zpipe_blocks(ldsp, TPC * G))),
+                         dim3(TPC * G), ldsp, s_, in, out, tw_.at(M), tw_.at(NZ), rows, kzs_, in_lim, scale, G,
                          groups);
       return;
     }

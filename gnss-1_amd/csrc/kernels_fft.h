@@ -177,10 +177,12 @@ struct ZRow {
   static constexpr int RS = Pad::stride(M);
   static_assert(64 % TPC == 0, "a z row must lie inside one wavefront");
 
-  // Registers of one spectral row as loaded from HBM (entries k >= lim read as 0).  A C2R ignores
-  // the imaginary parts of the entries k = 0 and k = M, so thread 0 keeps Re X[M] in x[0].y.
+  // Registers of one spectral row as loaded from HBM (entries k >= lim read as 0); thread 0 also
+  // holds Re X[M] in nyq (a register of its own: loaded into half of x[0], it would have to wait
+  // for the load of x[0] before it could be issued).
   struct Raw {
     cx<T> x[E];
+    T nyq;
   };
 
   __device__ __forceinline__ static void load(const cx<T>* __restrict__ src, bool active, int lim, int t,
@@ -190,32 +192,41 @@ struct ZRow {
       const int k = t + e * TPC;
       r.x[e] = (active && k < lim) ? ld_s(&src[(unsigned)k]) : mk<T>(0, 0);
     }
-    if (t == 0) r.x[0].y = (active && M < lim) ? src[M].x : (T)0;
+    r.nyq = (t == 0 && active && M < lim) ? src[M].x : (T)0;
   }
 
   // Row formed on loading: SA A + SB i kz B (SA in {-1, 0, 1}, SB in {-1, 1}), entries k >= lim 0;
-  // kzr[e]: kz of this thread's position k = t + e TPC, kz: the table (for k = M).
+  // kzr[e]: kz of this thread's position k = t + e TPC, kz: the table (for k = M).  Every load of
+  // the row is issued before any value is formed: forming a value inside its load's predicate would
+  // wait for that load alone, one HBM round trip per element.
   template <int SA, int SB>
   __device__ __forceinline__ static void load_ikz(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
                                                   const T* kzr, const T* __restrict__ kz, bool active, int lim,
                                                   int t, Raw& r) {
-    auto at = [&](unsigned k, T kzk) {
-      const cx<T> b = ld_s(&B[k]);
+    auto form = [&](const cx<T>& b, const cx<T>& x, T kzk) {
       const T kk = (T)SB * kzk;
       cx<T> v = mk<T>(-kk * b.y, kk * b.x);
       if (SA != 0) {
-        const cx<T> x = ld_s(&A[k]);
         v.x += (T)SA * x.x;
         v.y += (T)SA * x.y;
       }
       return v;
     };
+    cx<T> bv[E], av[SA != 0 ? E : 1];
+    bool on[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + e * TPC;
-      r.x[e] = (active && k < lim) ? at((unsigned)k, kzr[e]) : mk<T>(0, 0);
+      on[e] = active && k < lim;
+      bv[e] = on[e] ? ld_s(&B[(unsigned)k]) : mk<T>(0, 0);
+      if (SA != 0) av[e] = on[e] ? ld_s(&A[(unsigned)k]) : mk<T>(0, 0);
     }
-    if (t == 0) r.x[0].y = (active && M < lim) ? at(M, kz[M]).x : (T)0;
+    const bool nq = t == 0 && active && M < lim;
+    const cx<T> bm = nq ? ld_s(&B[M]) : mk<T>(0, 0);
+    const cx<T> am = (nq && SA != 0) ? ld_s(&A[M]) : mk<T>(0, 0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) r.x[e] = on[e] ? form(bv[e], av[SA != 0 ? e : 0], kzr[e]) : mk<T>(0, 0);
+    r.nyq = nq ? form(bm, am, kz[M]).x : (T)0;
   }
 
   // Twiddles of one thread for every row it transforms, read once and kept in registers: the
@@ -262,7 +273,7 @@ struct ZRow {
       cx<T> x[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = r[b].x[e];
-      const T xm = x[0].y;  // Re X[M] in thread 0
+      const T xm = r[b].nyq;  // Re X[M] in thread 0 (a C2R ignores Im X[0] and Im X[M])
       x[0].y = t0 ? (T)0 : x[0].y;
       cx<T> xc[E];
 #pragma unroll
@@ -385,7 +396,20 @@ k_c2r_z(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __restri
 // Persistent, software-pipelined forms of k_r2c_z / k_c2r_z: each block walks row groups
 // blockIdx.x, blockIdx.x + gridDim.x, ... and issues the HBM loads of its next rows before
 // transforming the current ones.  Every row lives in one wavefront (wave-level LDS sync only),
-// so the waves of a block never wait for each other.
+// so the waves of a block never wait for each other.  Both twiddle tables are copied to LDS after
+// the G row regions (M entries each): read from memory inside the loop, every twiddle wait would
+// also wait for the next rows' loads.
+template <typename T, int M>
+__device__ __forceinline__ cx<T>* zpipe_twiddles(cx<T>* lds, int G, int RS, const cx<T>* __restrict__ twM,
+                                                 const cx<T>* __restrict__ twN) {
+  cx<T>* tl = lds + (long long)G * RS;
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    tl[i] = twM[i];
+    tl[M + i] = twN[i];
+  }
+  __syncthreads();
+  return tl;
+}
 template <typename T, int NZ>
 __global__ void __launch_bounds__(lb_threads((NZ / 2) / Plan<NZ / 2>::E))
 k_r2c_z_pipe(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ twM,
@@ -405,6 +429,7 @@ k_r2c_z_pipe(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __r
   long long grp = blockIdx.x;
   cx<T> v[1][Z::E], nv[Z::E];
   load(grp, v[0]);
+  const cx<T>* tl = zpipe_twiddles<T, Z::M>(lds, G, Z::RS, twM, twN);
   while (true) {
     const long long next = grp + gridDim.x;
     const bool more = next < ngroups;
@@ -412,7 +437,7 @@ k_r2c_z_pipe(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __r
     const long long row = grp * G + g;
     const bool active = row < nrows;
     cx<T>* dst[1] = {out + (active ? row : 0) * kzs};
-    Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, twM, twN);
+    Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, tl, tl + Z::M);
     if (!more) break;
 #pragma unroll
     for (int e = 0; e < Z::E; ++e) v[0][e] = nv[e];
@@ -436,6 +461,7 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
     const long long row = grp * G + g;
     Z::load(in + (row < nrows ? row : 0) * kzs, row < nrows, in_lim, t, r[0]);
   }
+  const cx<T>* tl = zpipe_twiddles<T, Z::M>(lds, G, Z::RS, twM, twN);
   while (true) {
     const long long next = grp + gridDim.x;
     const bool more = next < ngroups;
@@ -446,7 +472,7 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
     const long long row = grp * G + g;
     const bool active = row < nrows;
     cx<T> v[1][Z::E];
-    Z::template c2r<1>(r, lds, LdsRow{g * Z::RS}, t, twM, twN, scale, v);
+    Z::template c2r<1>(r, lds, LdsRow{g * Z::RS}, t, tl, tl + Z::M, scale, v);
     if (active) {
       cx<T>* dst = reinterpret_cast<cx<T>*>(out + row * NZ);
 #pragma unroll
