@@ -1667,11 +1667,11 @@ class Engine : public EngineBase {
   template <int L, int CC, int NIN, int NOUT, int FORM>
   void pair_launch(const PairList<T>& li, int n, ColArgs a, const T* kv) {
     constexpr int TPC = L / Plan<L>::E;
-    // two batched outputs use two LDS regions, and a two-output kernel keeps the twiddles in LDS:
-    // up to the 160 KiB a gfx950 workgroup can hold
+    // two batched outputs use two LDS regions, and the twiddles have a copy after them: up to the
+    // 160 KiB a gfx950 workgroup can hold
     constexpr long long PAIR_LDS_MAX = 160 * 1024;
     const long long region = (long long)(NOUT == 2 && FORM == PAIR_SEQ ? 1 : NOUT) * L * CC * (long long)sizeof(C);
-    const long long twb = NOUT == 2 ? (long long)L * (long long)sizeof(C) : 0;
+    const long long twb = (long long)L * (long long)sizeof(C);
     int G = std::max(1, 256 / (CC * TPC));
     while (G > 1 && G * region + twb > PAIR_LDS_MAX) --G;
     if (region + twb > PAIR_LDS_MAX) throw Error(GNSS_ERR_UNSUPPORTED, "late pass tile does not fit the LDS");

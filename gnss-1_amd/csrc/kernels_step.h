@@ -681,8 +681,8 @@ constexpr int PAIR_BATCHED = 0, PAIR_SEQ = 1, PAIR_BATCHED_PF = 2;
 // Persistent like k_xinv_op_pipe: tile i is tile i % tpf of list entry i / tpf; B is loaded when
 // the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch; FORM: how two
 // outputs share the block (one output: always with the next tile's A loaded before the transform).
-// The twiddles of a two-output kernel are read from a copy in LDS, after the data regions: read
-// from memory, each stage would wait for every load in flight, the next tile's included.  An entry
+// The twiddles are read from a copy in LDS, after the data regions: read from memory, each radix
+// stage would wait for every load in flight, the next tile's included.  An entry
 // may write its outputs over its own inputs (each tile is read before it is written, by the one
 // block that owns it, and both outputs are formed from registers); no entry may read a field
 // another entry writes.
@@ -750,13 +750,9 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
   int tile = blockIdx.x;
   cx<T> va[E];
   if (PF) load_col(li.A[tile / tpf], tile, va);
-  const cx<T>* twl = tw;
-  if constexpr (NOUT == 2) {
-    cx<T>* tws = lds + (long long)a.G * NREG * L * C;
-    for (int i = tid; i < L; i += blockDim.x) tws[i] = tw[i];
-    __syncthreads();
-    twl = tws;
-  }
+  cx<T>* twl = lds + (long long)a.G * NREG * L * C;
+  for (int i = tid; i < L; i += blockDim.x) twl[i] = tw[i];
+  __syncthreads();
   while (true) {
     const int ent = tile / tpf;
     const PairOp op = li.op[ent];

@@ -37,13 +37,15 @@ def test_batched_ensemble_equals_unbatched(g, engine, monkeypatch, tmp_path):
     pass for all of them); with GNSS_ENSEMBLE_BATCH=0 every member has an engine of its own.  With
     the same pass variants (a table without the batched "<key>b" entries) the per-element
     arithmetic is the same, so fields and statistics agree bit for bit; with the shipped table the
-    batched engines take their own variants, which agree to FP32 rounding."""
+    batched engines take their own variants, which agree to FP32 rounding.  A batched engine always
+    runs the late-derivative x mode (xf = 4), so the plain table gives the stand-alone engines that
+    mode too."""
     import json
     from conftest import PKG
     table = json.load(open(f"{PKG}/tuning/gnss_tuning.json"))
     plain = tmp_path / "plain.json"
-    plain.write_text("{\n" + ",\n".join(f'"{k}": {json.dumps(v)}' for k, v in table.items()
-                                          if not k.endswith("b")) + "\n}\n")
+    plain.write_text("{\n" + ",\n".join(f'"{k}": {json.dumps(v if k.endswith("c") else dict(v, xf=4))}'
+                                          for k, v in table.items() if not k.endswith("b")) + "\n}\n")
     cfgs = g.ensemble_configs(counts=((64, 5), (96, 3), (128, 2)))
 
     def run(batch, tuning):
