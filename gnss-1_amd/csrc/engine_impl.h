@@ -715,7 +715,7 @@ class Engine : public EngineBase {
     {
       int best = zf_;
       double best_ms = 1e30;
-      for (int v = 0; v <= 8; ++v) {
+      for (int v = 0; v <= 9; ++v) {
         if (!zvariant_ok(v)) continue;
         zf_ = v;
         const double ms = time_it([&] { z_fused(); });
@@ -849,7 +849,7 @@ class Engine : public EngineBase {
       {
         int best = zf_;
         double best_ms = 1e30;
-        for (int v = 0; v <= 8; ++v) {
+        for (int v = 0; v <= 9; ++v) {
           if (!zvariant_ok(v)) continue;
           zf_ = v;
           const double ms = time_it([&] { z_fused(); });
@@ -924,7 +924,9 @@ class Engine : public EngineBase {
 
  private:
   // fused z variants built: FP32 0-7 (8 is 7 in FP32), FP64 0, 3, 6, 7 and 8
-  static bool zvariant_ok(int v) {
+  // fused z variants built for this context (9, two rows per transform: no colloid, nz <= 512)
+  bool zvariant_ok(int v) const {
+    if (v == 9) return !colloid_ && nz_ <= 512;
     return sizeof(T) == 4 ? (v >= 0 && v <= 7) : (v == 0 || v == 3 || (v >= 6 && v <= 8));
   }
   // column tiles the fused x kernels are built for: FP64 4 or 8, FP32 8 or 16
@@ -1412,6 +1414,9 @@ class Engine : public EngineBase {
     if constexpr (sizeof(T) == 8) {
       if (zf_ == 8) return zfused_launch_v<NZ, 8>(a0, scale);
     }
+    if constexpr (NZ <= 512) {
+      if (zf_ == 9) return zfused_launch_v<NZ, 9>(a0, scale);
+    }
     zfused_launch_v<NZ, 0>(a0, scale);
   }
 
@@ -1421,7 +1426,9 @@ class Engine : public EngineBase {
     using P = typename FusedPlan<T, M, V>::type;
     constexpr int TPC = M / P::E;
     int G = std::max(1, 256 / TPC);
-    const long long row_lds = (long long)FUSED_B * FusedPlan<T, M, V>::pad::stride(M) * (long long)sizeof(C);
+    // LDS regions of one row: three fields of length M per radix stage, or (V = 9) PAIR_B rows of NZ
+    const long long row_lds = V == 9 ? (long long)PAIR_B * FusedPlan<T, M, V>::pad::stride(NZ) * (long long)sizeof(C)
+                                     : (long long)FUSED_B * FusedPlan<T, M, V>::pad::stride(M) * (long long)sizeof(C);
     while (G > 1 && G * row_lds > LDS_MAX) --G;
     ZFusedArgs<T> a = a0;
     a.G = G;

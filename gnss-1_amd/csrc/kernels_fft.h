@@ -545,6 +545,23 @@ template <typename T, int M> struct FusedPlan<T, M, 7> : FusedPlan<T, M, 6> {
 template <typename T, int M> struct FusedPlan<T, M, 8> : FusedPlan<T, M, 7> {
   using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<3, 2>>::type;
 };
+// Half spectra of the variant below as loaded: positions k = t + e TPC (e < E/2) of the full-length
+// plan's threads, i.e. half its elements per thread with the same threads per row.
+template <int NZ> struct HalfPlan {
+  static constexpr int E = Plan<NZ>::E / 2;
+  using R = typename Plan<NZ>::R;
+};
+// V = 9: two real rows per complex transform of the full row length (z = a + i b), eight elements
+// per thread in three radix-8 stages at 512 (two LDS exchanges instead of three, and no pairing of
+// k with M - k per row); inverse 1/N on the forward side as in 7; twiddles in registers.
+template <typename T, int M> struct FusedPlan<T, M, 9> {
+  using type = HalfPlan<2 * M>;
+  static constexpr bool prefetch = false;
+  using pad = typename std::conditional<sizeof(T) == 4, LdsRowP<5, 5>, LdsRowP<4, 3>>::type;
+  static constexpr int min_waves = 1;
+  static constexpr bool regtw = true;
+  static constexpr bool defer = true;
+};
 
 
 // kz entries a substage's rows hold: kz <= nz/3 (the 2/3 rule), rounded up to the widest column tile
@@ -704,6 +721,187 @@ __device__ __forceinline__ void zfused_row(const ZFusedArgs<T>& a, const TM& twM
   }
 }
 
+// Two real z rows a, b of length N = NZ as one complex row z = a + i b through one N-point transform.
+// The C2R side builds Z = A + i B at all N positions from the two loaded half spectra (A and B are
+// Hermitian: Z[N - k] = conj(A[k]) + i conj(B[k]), the partner entries a lane shuffle away); the
+// R2C side splits Z = FFT(p1 + i p2) into P1[k] = (Z[k] + conj Z[N - k]) / 2 and
+// P2[k] = (Z[k] - conj Z[N - k]) / 2i.  Positions k = t + e TPC, e < E/2, are the half spectrum;
+// the partner N - k of a position in the upper half lives in lane TPC - t, register E/2 - 1 - j
+// (j = e - E/2), or, in lane 0, in its own register E/2 - j (the Nyquist entry for j = 0).
+template <typename T, int NZ, typename Pad>
+struct ZPair {
+  using P = Plan<NZ>;
+  using H = ZRow<T, NZ, HalfPlan<NZ>>;  // loaders of one half spectrum (k < N/2, Nyquist in nyq)
+  using Raw = typename H::Raw;
+  static constexpr int N = NZ;
+  static constexpr int E = P::E;
+  static constexpr int HE = E / 2;
+  static constexpr int TPC = N / E;
+  static constexpr int RS = Pad::stride(N);
+  static_assert(H::TPC == TPC && H::E == HE, "half rows and full rows must share their threads");
+  static_assert(64 % TPC == 0, "a z row must lie inside one wavefront");
+
+  __device__ __forceinline__ static cx<T> mirror(const cx<T>& x, int t) {
+    const int src = (TPC - t) & (TPC - 1);
+    cx<T> r;
+    r.x = __shfl(x.x, src, TPC);
+    r.y = __shfl(x.y, src, TPC);
+    return r;
+  }
+  // Z at all positions from the half spectra ra, rb (their Im X[0] and Im X[N/2] ignored)
+  __device__ __forceinline__ static void pack(const Raw& ra, const Raw& rb, int t, cx<T>* z) {
+    const bool t0 = t == 0;
+    cx<T> xa[HE], xb[HE];
+#pragma unroll
+    for (int e = 0; e < HE; ++e) {
+      xa[e] = ra.x[e];
+      xb[e] = rb.x[e];
+    }
+    xa[0].y = t0 ? (T)0 : xa[0].y;
+    xb[0].y = t0 ? (T)0 : xb[0].y;
+    cx<T> pa[HE], pb[HE];
+#pragma unroll
+    for (int j = 0; j < HE; ++j) {
+      const cx<T> oa = j == 0 ? mk<T>(ra.nyq, (T)0) : xa[HE - j];
+      const cx<T> ob = j == 0 ? mk<T>(rb.nyq, (T)0) : xb[HE - j];
+      cx<T> sa = xa[HE - 1 - j], sb = xb[HE - 1 - j];
+      sa.x = t0 ? oa.x : sa.x;
+      sa.y = t0 ? oa.y : sa.y;
+      sb.x = t0 ? ob.x : sb.x;
+      sb.y = t0 ? ob.y : sb.y;
+      pa[j] = mirror(sa, t);
+      pb[j] = mirror(sb, t);
+    }
+#pragma unroll
+    for (int e = 0; e < HE; ++e) z[e] = mk<T>(xa[e].x - xb[e].y, xa[e].y + xb[e].x);
+#pragma unroll
+    for (int j = 0; j < HE; ++j) z[HE + j] = mk<T>(pa[j].x + pb[j].y, pb[j].x - pa[j].y);
+  }
+  // P1, P2 of Z (after the forward transform) times 2h; entries k < lim stored to d1 / d2 (d2 may be
+  // null: a lone row), with the Nyquist value at k = N/2 and zero padding up to kzs
+  __device__ __forceinline__ static void unpack_store(const cx<T>* z, cx<T>* d1, cx<T>* d2, bool active, int lim,
+                                                      int kzs, int t, T h) {
+    const bool t0 = t == 0;
+    cx<T> zm[HE];
+#pragma unroll
+    for (int e = 0; e < HE; ++e) {
+      const cx<T> own = z[(E - e) % E];
+      cx<T> snd = z[E - 1 - e];
+      snd.x = t0 ? own.x : snd.x;
+      snd.y = t0 ? own.y : snd.y;
+      zm[e] = mirror(snd, t);
+    }
+    if (!active) return;
+#pragma unroll
+    for (int e = 0; e < HE; ++e) {
+      const int k = t + e * TPC;
+      if (k < lim) {
+        st_s(&d1[(unsigned)k], mk<T>(h * (z[e].x + zm[e].x), h * (z[e].y - zm[e].y)));
+        if (d2) st_s(&d2[(unsigned)k], mk<T>(h * (z[e].y + zm[e].y), h * (zm[e].x - z[e].x)));
+      }
+    }
+    if (t0 && N / 2 < lim) {
+      st_s(&d1[N / 2], mk<T>(((T)2 * h) * z[HE].x, (T)0));
+      if (d2) st_s(&d2[N / 2], mk<T>(((T)2 * h) * z[HE].y, (T)0));
+    }
+    for (int k = N / 2 + 1 + t; k < kzs && k < lim; k += TPC) {
+      st_s(&d1[k], mk<T>(0, 0));
+      if (d2) st_s(&d2[k], mk<T>(0, 0));
+    }
+  }
+};
+
+// Fused z pass in pairs of rows (variant 9, no colloid): C2R of (u, v), (w, omega_x),
+// (omega_y, omega_z), then per scalar (phi_x, phi_y), (phi_z, 0); R2C of the products paired in
+// order ((u x omega)_x, (u x omega)_y), ((u x omega)_z, u.grad phi_0), (u.grad phi_1, ...) ...
+constexpr int PAIR_B = 2;  // packed rows per radix-stage batch (LDS regions per row of the block)
+template <typename T, int NZ, typename Pad, typename TW>
+__device__ __forceinline__ void zfused_pair_row(const ZFusedArgs<T>& a, const TW& tw, T scale, cx<T>* lds, int t,
+                                                int g) {
+  using Z = ZPair<T, NZ, Pad>;
+  using H = typename Z::H;
+  using P = typename Z::P;
+  constexpr int E = Z::E, HE = Z::HE, N = Z::N, RS = Z::RS;
+  constexpr int LIM = zfused_lim<T>(NZ);
+  const T h = (T)0.5 * scale * scale;  // the inverse transforms' 1/N taken here as 1/N^2
+  const long long row = (long long)blockIdx.x * a.G + g;
+  const bool active = row < a.nrows;
+  const long long off = (active ? row : 0) * a.kzs;
+  const Pad idx{g * RS * PAIR_B};
+  T kzr[HE];
+#pragma unroll
+  for (int e = 0; e < HE; ++e) kzr[e] = (a.late && t + e * Z::TPC < LIM) ? a.kz[t + e * Z::TPC] : (T)0;
+  typename Z::Raw r[4];
+  // velocity and vorticity: Z1 = u + i v, Z2 = w + i omega_x, then Z3 = omega_y + i omega_z
+  H::load(a.ws[0] + off, active, LIM, t, r[0]);
+  H::load(a.ws[1] + off, active, LIM, t, r[1]);
+  H::load(a.ws[2] + off, active, LIM, t, r[2]);
+  if (a.late) H::template load_ikz<1, -1>(a.ws[3] + off, a.ws[1] + off, kzr, a.kz, active, LIM, t, r[3]);
+  else H::load(a.ws[3] + off, active, LIM, t, r[3]);
+  cx<T> za[2][E];
+  Z::pack(r[0], r[1], t, za[0]);
+  Z::pack(r[2], r[3], t, za[1]);
+  lds_sync<true>();
+  fft_batch<T, N, +1, Pad, P, true, 2, RS>(za, lds, t, idx, tw);
+  if (a.late) H::template load_ikz<-1, 1>(a.ws[4] + off, a.ws[0] + off, kzr, a.kz, active, LIM, t, r[0]);
+  else H::load(a.ws[4] + off, active, LIM, t, r[0]);
+  H::load(a.ws[5] + off, active, LIM, t, r[1]);
+  cx<T> zb[1][E];
+  Z::pack(r[0], r[1], t, zb[0]);
+  lds_sync<true>();
+  fft_batch<T, N, +1, Pad, P, true, 1, RS>(zb, lds, t, idx, tw);
+  // u x omega; the velocity stays in za[0] (u + i v) and w
+  T w[E], pend[E];
+  cx<T> q[1][E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T u = za[0][e].x, v = za[0][e].y, ww = za[1][e].x;
+    const T ox = za[1][e].y, oy = zb[0][e].x, oz = zb[0][e].y;
+    q[0][e] = mk<T>(v * oz - ww * oy, ww * ox - u * oz);
+    pend[e] = u * oy - v * ox;
+    w[e] = ww;
+  }
+  lds_sync<true>();
+  fft_batch<T, N, -1, Pad, P, true, 1, RS>(q, lds, t, idx, tw);
+  Z::unpack_store(q[0], a.nl[0] + off, a.nl[1] + off, active, LIM, a.kzs, t, h);
+  int pend_f = 2;  // product row waiting for a partner
+  const typename Z::Raw zero{};
+  for (int s = 0; s < a.S; ++s) {
+    const int w0 = 6 + 3 * s;
+    H::load(a.ws[w0] + off, active, LIM, t, r[0]);
+    H::load(a.ws[w0 + 1] + off, active, LIM, t, r[1]);
+    if (a.late) H::template load_ikz<0, 1>(a.ws[w0 + 2] + off, a.ws[w0 + 2] + off, kzr, a.kz, active, LIM, t, r[2]);
+    else H::load(a.ws[w0 + 2] + off, active, LIM, t, r[2]);
+    cx<T> zg[2][E];
+    Z::pack(r[0], r[1], t, zg[0]);
+    Z::pack(r[2], zero, t, zg[1]);
+    lds_sync<true>();
+    fft_batch<T, N, +1, Pad, P, true, 2, RS>(zg, lds, t, idx, tw);
+    T n[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) n[e] = za[0][e].x * zg[0][e].x + za[0][e].y * zg[0][e].y + w[e] * zg[1][e].x;
+    if (pend_f >= 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) q[0][e] = mk<T>(pend[e], n[e]);
+      lds_sync<true>();
+      fft_batch<T, N, -1, Pad, P, true, 1, RS>(q, lds, t, idx, tw);
+      Z::unpack_store(q[0], a.nl[pend_f] + off, a.nl[3 + s] + off, active, LIM, a.kzs, t, h);
+      pend_f = -1;
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) pend[e] = n[e];
+      pend_f = 3 + s;
+    }
+  }
+  if (pend_f >= 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) q[0][e] = mk<T>(pend[e], (T)0);
+    lds_sync<true>();
+    fft_batch<T, N, -1, Pad, P, true, 1, RS>(q, lds, t, idx, tw);
+    Z::unpack_store(q[0], a.nl[pend_f] + off, (cx<T>*)nullptr, active, LIM, a.kzs, t, h);
+  }
+}
+
 template <typename T, int NZ, int V>
 __global__ void __launch_bounds__(lb_threads((NZ / 2) / FusedPlan<T, NZ / 2, V>::type::E),
                                   (FusedPlan<T, NZ / 2, V>::min_waves))
@@ -714,7 +912,11 @@ k_zfused(ZFusedArgs<T> a, const cx<T>* __restrict__ twM, const cx<T>* __restrict
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int t = threadIdx.x % Z::TPC;
   const int g = Z::TPC == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x / 64) : threadIdx.x / Z::TPC;
-  if constexpr (FP::regtw) {
+  if constexpr (V == 9) {
+    TwRegs<T, NZ, Plan<NZ>> tw;
+    tw.load(twN, t);
+    zfused_pair_row<T, NZ, typename FP::pad>(a, tw, scale, lds, t, g);
+  } else if constexpr (FP::regtw) {
     typename Z::Tw tw;
     tw.load(twM, twN, t);
     zfused_row<T, NZ, V>(a, tw, tw, scale, lds, t, g);

@@ -171,6 +171,33 @@ def test_fused_register_twiddles(g, engine, prec, nz):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("nz", [16, 24, 48, 96, 192, 384, 512])
+def test_fused_pair_rows_match(g, engine, prec, nz):
+    """Fused z variant 9 transforms two real rows per complex transform of the full row length
+    (z = a + i b; the half spectra packed with their Hermitian partners by lane shuffles, the
+    products split again after the forward transform): the same step as variant 7, to rounding
+    (FP64 1e-13, FP32 2e-6), in the late-derivative and the work-field x modes, with zero, one
+    and two scalars (an odd or even number of product rows), and refused with a colloid."""
+    print(f"\n[pair rows nz={nz} prec={prec}]", flush=True)
+    for sc in ((), (0.7,), (0.7, 2.0)):
+        for xf in (2, 5):
+            cfg = g.Config(n=(32, 16, nz), re=400.0, sc=sc, theta=0.08, noise_kmax=6.0, noise_amp=0.2, dt=1e-3,
+                           precision=prec)
+            out = []
+            for zf in (7, 9):
+                a = g.Solver(cfg, engine).setup()
+                a.set_tuning(f"zf={zf} xf={xf}")
+                a.step(2)
+                out.append([a.get_spectral(f) for f in range(3 + len(sc))])
+                a.close()
+            for f in range(3 + len(sc)):
+                assert rel(out[1][f], out[0][f]) <= (1e-13 if prec == 0 else 2e-6), (sc, xf, f)
+    c = g.Solver(g.with_colloid(g.Config(n=(32, 16, nz), sc=(0.7, 2.0), precision=prec)), engine)
+    with pytest.raises(g.GnssError):
+        c.set_tuning("zf=9")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_per_field_x_inverse_matches(g, engine, prec):
     """x modes 2 and 3 form each work field inside its x inverse pass (u, omega, grad phi, the
     colloid's c and lap Z) with the operations of the prologue (mode 0) or of the group-fused
