@@ -9,8 +9,8 @@ all members per launch (`gnss_ensemble_autotune`), giving the "<key>b" entries. 
 kp keeps the value the table already holds (0 by default).
 
 The table is updated in place: keys this run tunes are replaced, every other entry is kept.  The
-raw candidate timings go to --report (default gpurun_out/autotune_report.json), not beside the
-table.
+raw candidate timings go to --report (default autotune_report.json in the working directory, an
+untracked file), not beside the table.
 
 usage: python gnss-1_amd/autotune.py [--reps 5] [--out gnss-1_amd/tuning/gnss_tuning.json]
                                      [--big] [--only KEY ...] [--ensemble | --ensemble-only]
@@ -101,7 +101,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning", "gnss_tuning.json"))
-    ap.add_argument("--report", default=os.path.join("gpurun_out", "autotune_report.json"))
+    ap.add_argument("--report", default="autotune_report.json")
     ap.add_argument("--big", action="store_true", help="also tune the 1024^3 FP32 workload")
     ap.add_argument("--ensemble", action="store_true", help="also tune the batched ensemble engines")
     ap.add_argument("--ensemble-only", action="store_true", help="tune only the batched ensemble engines")
