@@ -1147,13 +1147,15 @@ class Engine : public EngineBase {
     const size_t lds = (size_t)G * L * CC * sizeof(C);
     const long long blocks = (long long)a.nchunks * cdiv(a.nouter, G);
     if (cp_) {
+      // the persistent form also holds the twiddle table in LDS, after the data regions
+      const size_t ldsp = lds + (size_t)L * sizeof(C);
       // persistent: as many blocks as stay resident (LDS and wave limits of a CU), all CUs
-      const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)lds,
+      const long long per_cu = std::max<long long>(1, std::min<long long>(160 * 1024 / (long long)ldsp,
                                                                            2048 / threads));
       const long long grid = std::min<long long>(blocks * nf, per_cu * n_cu_);
       FieldList<T> fl{};
       for (int f = 0; f < nf; ++f) fl.p[f] = fields[f];
-      hipLaunchKernelGGL((k_col_multi<T, L, SIGN, CC>), dim3((unsigned)grid), dim3(threads), lds, s_, fl,
+      hipLaunchKernelGGL((k_col_multi<T, L, SIGN, CC>), dim3((unsigned)grid), dim3(threads), ldsp, s_, fl,
                          tw_.at(L), a, (int)blocks, (int)(blocks * nf));
       return;
     }

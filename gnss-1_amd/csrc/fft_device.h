@@ -75,6 +75,14 @@ template <typename V> __device__ __forceinline__ void st_at(V* base, unsigned by
   st_s(reinterpret_cast<V*>(reinterpret_cast<char*>(base) + byte_off), v);
 }
 
+// Waits until none of this wave's vector-memory operations is in flight: s_waitcnt vmcnt(0), with
+// the expcnt (bits 6:4) and lgkmcnt (bits 11:8) fields at their maxima so that only vmcnt (bits
+// 3:0 and 15:14) is waited for.  Before a pipelined loop: loads issued ahead of it on a path that may
+// skip some of them (a loop of unknown trip count, a lane predicate) would otherwise leave the
+// compiler unable to count the loads in flight inside the loop, and its first wait there would be
+// for every load, the next tile's prefetch included.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // multiply by (SIGN * i)
 template <int SIGN, typename V> __device__ __forceinline__ V cmul_si(V a) {
   V r;

@@ -140,12 +140,26 @@ k_col_multi(FieldList<T> fl, const cx<T>* __restrict__ tw, ColArgs a, int tpf, i
   int tile = blockIdx.x;
   cx<T> v[E], nv[E];
   load(tile, v);
+  // The twiddles are read from a copy in LDS after the G column regions.  Held in registers from
+  // loads before the loop, their first use inside it had to wait for every later load, the next
+  // tile's included (the prefetch may be skipped, so the compiler cannot count it); the first
+  // tile is waited for here for the same reason.
+  cx<T>* twl = lds + (long long)a.G * L * C;
+  for (int i = tid; i < L; i += blockDim.x) twl[i] = tw[i];
+  wait_vmem();
+  __syncthreads();
   const LdsCol idx{g * L * C + c, C};
   while (true) {
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
     if (more) load(next, nv);
-    fft<T, L, SIGN>(v, lds, t, idx, tw);
+    fft<T, L, SIGN>(v, lds, t, idx, twl);
+    // One wait per tile, here: the vector-memory counter covers loads and stores alike and, with
+    // both in flight, every wait is a wait for all of them.  Here the next tile's loads have had
+    // the whole transform to arrive and this tile's stores are not yet issued; left to the
+    // compiler, the wait came after the stores (for the copy of the next tile), exposing their
+    // full latency once per tile.
+    wait_vmem();
     bool active;
     cx<T>* base = base_of(tile, active);
     if (active) {
@@ -313,9 +327,12 @@ struct ZRow {
   template <int B, typename TM, typename TN>
   __device__ __forceinline__ static void r2c(cx<T> (*v)[E], cx<T>* const* dst, bool active, int lim,
                                              int kzs, cx<T>* lds, const Pad& idx, int t, const TM& twM,
-                                             const TN& twN, T h = (T)0.5) {
+                                             const TN& twN, T h = (T)0.5, bool drain = false) {
     lds_sync<true>();
     fft_batch<T, M, -1, Pad, P, true, B, RS>(v, lds, t, idx, mtw(twM, twN));
+    // drain: wait for every memory operation in flight before the stores (a pipelined caller's
+    // next rows, loaded before the transform; see wait_vmem)
+    if (drain) wait_vmem();
     const bool t0 = t == 0;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -407,6 +424,7 @@ __device__ __forceinline__ cx<T>* zpipe_twiddles(cx<T>* lds, int G, int RS, cons
     tl[i] = twM[i];
     tl[M + i] = twN[i];
   }
+  wait_vmem();  // the first rows too: the loop then starts with no load of unknown count in flight
   __syncthreads();
   return tl;
 }
@@ -437,7 +455,7 @@ k_r2c_z_pipe(const T* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __r
     const long long row = grp * G + g;
     const bool active = row < nrows;
     cx<T>* dst[1] = {out + (active ? row : 0) * kzs};
-    Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, tl, tl + Z::M);
+    Z::template r2c<1>(v, dst, active, out_lim, kzs, lds, LdsRow{g * Z::RS}, t, tl, tl + Z::M, (T)0.5, true);
     if (!more) break;
 #pragma unroll
     for (int e = 0; e < Z::E; ++e) v[0][e] = nv[e];
@@ -473,6 +491,7 @@ k_c2r_z_pipe(const cx<T>* __restrict__ in, T* __restrict__ out, const cx<T>* __r
     const bool active = row < nrows;
     cx<T> v[1][Z::E];
     Z::template c2r<1>(r, lds, LdsRow{g * Z::RS}, t, tl, tl + Z::M, scale, v);
+    wait_vmem();  // the next rows, before this group's stores (see wait_vmem)
     if (active) {
       cx<T>* dst = reinterpret_cast<cx<T>*>(out + row * NZ);
 #pragma unroll

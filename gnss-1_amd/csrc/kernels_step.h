@@ -732,10 +732,14 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
   const LdsCol idx{g * NREG * L * C + c, C};
   constexpr bool PF = NOUT == 1 || FORM != PAIR_BATCHED;
   // output j of entry op at this thread's positions
+  // the wavenumbers of this thread's rows, loaded once before the loop (see wait_vmem below)
+  T kr[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) kr[e] = kv[t + e * TPC];
   auto form = [&](const PairOp& op, int j, const cx<T>* va, const cx<T>* vb, cx<T>* r) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const T k = kv[t + e * TPC];
+      const T k = kr[e];
       const T ka = (T)op.b[j] * k, ra = (T)op.a[j];
       cx<T> x = mk<T>(ra * va[e].x - ka * va[e].y, ra * va[e].y + ka * va[e].x);
       if (NIN == 2) {
@@ -752,6 +756,10 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
   if (PF) load_col(li.A[tile / tpf], tile, va);
   cx<T>* twl = lds + (long long)a.G * NREG * L * C;
   for (int i = tid; i < L; i += blockDim.x) twl[i] = tw[i];
+  // Nothing in flight when the loop starts: loads issued on paths that may skip some of them (the
+  // copy loop above, lane predicates) leave the compiler unable to count the loads in flight, and
+  // its first wait inside the loop would then be for every load, the next tile's included.
+  wait_vmem();
   __syncthreads();
   while (true) {
     const int ent = tile / tpf;
@@ -770,6 +778,11 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
         form(op, j, va, vb, v[0]);
         if (j == NOUT - 1 && more) load_col(li.A[next / tpf], next, va);
         fft_batch<T, L, +1, LdsCol, Plan<L>, false, 1, L * C>(v, lds, t, idx, twl);
+        // one wait per tile, after its last transform and before its last stores: the counter
+        // covers loads and stores alike, so any wait with both in flight waits for all of them;
+        // here the next tile's loads have had a whole transform to arrive, where the compiler's
+        // own wait came after the stores and exposed their latency once per tile
+        if (j == NOUT - 1) wait_vmem();
         if (active) {
           cx<T>* __restrict__ dst = li.out[j][ent];
 #pragma unroll
@@ -782,6 +795,7 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
       for (int j = 0; j < NOUT; ++j) form(op, j, va, vb, v[j]);
       if (PF && more) load_col(li.A[next / tpf], next, va);
       fft_batch<T, L, +1, LdsCol, Plan<L>, false, NOUT, L * C>(v, lds, t, idx, twl);
+      wait_vmem();  // as in the SEQ form
       if (active) {
 #pragma unroll
         for (int j = 0; j < NOUT; ++j) {
