@@ -678,9 +678,10 @@ constexpr unsigned removed_slabs(int L, int TPC, int E) {
 // next tile's A loaded before they transform (PAIR_BATCHED_PF).
 constexpr int PAIR_BATCHED = 0, PAIR_SEQ = 1, PAIR_BATCHED_PF = 2;
 
-// Persistent like k_xinv_op_pipe: tile i is tile i % tpf of list entry i / tpf; B is loaded when
-// the tile is formed.  NIN, NOUT: inputs and outputs of every entry of the launch; FORM: how two
-// outputs share the block (one output: always with the next tile's A loaded before the transform).
+// Persistent like k_xinv_op_pipe: tile i is tile i % tpf of list entry i / tpf; B is loaded with
+// A (one tile ahead in the prefetching forms).  NIN, NOUT: inputs and outputs of every entry of the
+// launch; FORM: how two outputs share the block (one output: always with the next tile loaded
+// before the transform).
 // The twiddles are read from a copy in LDS, after the data regions: read from memory, each radix
 // stage would wait for every load in flight, the next tile's included.  An entry
 // may write its outputs over its own inputs (each tile is read before it is written, by the one
@@ -752,8 +753,10 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     }
   };
   int tile = blockIdx.x;
-  cx<T> va[E];
+  // the prefetching forms load the next tile's B with its A
+  cx<T> va[E], vb[NIN == 2 ? E : 1];
   if (PF) load_col(li.A[tile / tpf], tile, va);
+  if (PF && NIN == 2) load_col(li.B[tile / tpf], tile, vb);
   cx<T>* twl = lds + (long long)a.G * NREG * L * C;
   for (int i = tid; i < L; i += blockDim.x) twl[i] = tw[i];
   // Nothing in flight when the loop starts: loads issued on paths that may skip some of them (the
@@ -765,8 +768,7 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
     const int ent = tile / tpf;
     const PairOp op = li.op[ent];
     if (!PF) load_col(li.A[ent], tile, va);
-    cx<T> vb[NIN == 2 ? E : 1];
-    if (NIN == 2) load_col(li.B[ent], tile, vb);
+    if (!PF && NIN == 2) load_col(li.B[ent], tile, vb);
     const int next = tile + gridDim.x;
     const bool more = next < ntiles;
     bool active;
@@ -776,7 +778,10 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
       for (int j = 0; j < NOUT; ++j) {
         cx<T> v[1][E];
         form(op, j, va, vb, v[0]);
-        if (j == NOUT - 1 && more) load_col(li.A[next / tpf], next, va);
+        if (j == NOUT - 1 && more) {
+          load_col(li.A[next / tpf], next, va);
+          if (NIN == 2) load_col(li.B[next / tpf], next, vb);
+        }
         fft_batch<T, L, +1, LdsCol, Plan<L>, false, 1, L * C>(v, lds, t, idx, twl);
         // one wait per tile, after its last transform and before its last stores: the counter
         // covers loads and stores alike, so any wait with both in flight waits for all of them;
@@ -793,7 +798,10 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
       cx<T> v[NOUT][E];
 #pragma unroll
       for (int j = 0; j < NOUT; ++j) form(op, j, va, vb, v[j]);
-      if (PF && more) load_col(li.A[next / tpf], next, va);
+      if (PF && more) {
+        load_col(li.A[next / tpf], next, va);
+        if (NIN == 2) load_col(li.B[next / tpf], next, vb);
+      }
       fft_batch<T, L, +1, LdsCol, Plan<L>, false, NOUT, L * C>(v, lds, t, idx, twl);
       wait_vmem();  // as in the SEQ form
       if (active) {
