@@ -1062,7 +1062,10 @@ class Engine : public EngineBase {
     dalloc(dkz_, sizeof(double) * nzh_);
     dalloc(tkx_, sizeof(T) * nx_);
     dalloc(tky_, sizeof(T) * ny_);
-    dalloc(tkz_, sizeof(T) * nzh_);
+    // the kz table padded with zeros to the row stride: the column tiles of the strided passes reach
+    // kz_lim, which on small grids lies a tile beyond nz/2 + 1 (those columns are masked)
+    dalloc(tkz_, sizeof(T) * kzs_);
+    GNSS_HIP(hipMemsetAsync(tkz_, 0, sizeof(T) * kzs_, s_));
     const int nb = n_bins();
     dalloc(d_part_, sizeof(double) * nx_ * SPEC_NQ);
     dalloc(d_spec_, sizeof(double) * (size_t)nx_ * (1 + S_) * nb);
@@ -1752,11 +1755,12 @@ class Engine : public EngineBase {
     constexpr int TPC = L / Plan<L>::E;
     const int nouter = 2 * g_.keep_y + 1;
     const long long blocks = (long long)nchunks * cdiv(nouter, G);
-    // the forward kernel keeps a copy of the twiddles after its data region, which the velocity
-    // group's right-hand sides may take over once the transforms are done
+    // the forward kernel keeps a copy of the twiddles and of the x wavenumbers after its data
+    // region, which the velocity group's right-hand sides may take over once the transforms are done
     const long long data = (long long)G * L * CC;
     const long long rhs = xfwd_rhs_entries<T, L, CC, GRP == 0>(CC * TPC * G);
-    const size_t lds = pro ? (size_t)data * sizeof(C) : (size_t)(std::max(data, rhs) + L) * sizeof(C);
+    const size_t lds = pro ? (size_t)data * sizeof(C)
+                           : (size_t)(std::max(data, rhs) + L) * sizeof(C) + (size_t)L * sizeof(T);
     if (lds > 160 * 1024) throw Error(GNSS_ERR_UNSUPPORTED, "fused x pass tile does not fit the LDS");
     if (!pro && g_.keep_x != L / 3) throw Error(GNSS_ERR_STATE, "fused x pass rows must be the kept ones");
     if (pro)
@@ -1787,7 +1791,7 @@ class Engine : public EngineBase {
     const int L = nx_, CC = xc_;
     const int TPC = L / elems_per_thread(L);
     int G = std::max(1, 256 / (CC * TPC));
-    const long long twb = pro ? 0 : (long long)L * (long long)sizeof(C);
+    const long long twb = pro ? 0 : (long long)L * (long long)(sizeof(C) + sizeof(T));
     while (G > 1 && (long long)G * L * CC * (long long)sizeof(C) + twb > LDS_MAX) --G;
     const int nchunks = kz_lim_ / CC;
     const int nouter = 2 * g_.keep_y + 1;

@@ -820,12 +820,14 @@ k_inv_pair_pipe(PairList<T> li, const T* __restrict__ kv, const cx<T>* __restric
 
 // The velocity group of the fused forward x pass parks its projected right-hand sides in LDS
 // (one slot per thread, field and element of a slab the 2/3 rule keeps) when they fit beside the
-// twiddles: its registers are then free to load the state of several elements before any store.
+// twiddles and the x wavenumbers: its registers are then free to load the state of several
+// elements before any store.
 // Entries of that region for blocks of `threads` threads (0: the sides stay in registers).
 template <typename T, int L, int C, bool VEL> __host__ __device__ constexpr long long xfwd_rhs_entries(int threads) {
   constexpr int E = Plan<L>::E, TPC = L / E;
   constexpr int NK = E - __builtin_popcount(removed_slabs(L, TPC, E));
-  constexpr long long most = (long long)NK * 3 * lb_threads(C * TPC) * (long long)sizeof(cx<T>) + L * (long long)sizeof(cx<T>);
+  constexpr long long most = (long long)NK * 3 * lb_threads(C * TPC) * (long long)sizeof(cx<T>) +
+                             L * (long long)(sizeof(cx<T>) + sizeof(T));
   return (VEL && most <= 160 * 1024) ? (long long)NK * 3 * threads : 0;
 }
 
@@ -870,9 +872,17 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
   for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int e = 0; e < E; ++e) nv[f][e] = active ? ld_at(nl.p[f0 + f] + at(e), lane_b) : mk<T>(0, 0);
+  // The x wavenumbers are read from a copy in LDS after the twiddles: loaded from memory per element
+  // inside the projection and the update, each was one more wait, and after the first stores a wait
+  // for every store in flight as well.
+  const T ky = kyv[iy], kz = kzv[iz];
   const long long rhs_n = xfwd_rhs_entries<T, L, C, VEL>(blockDim.x);
   cx<T>* tws = lds + ((long long)G * L * C > rhs_n ? (long long)G * L * C : rhs_n);
-  for (int i = tid; i < L; i += blockDim.x) tws[i] = tw[i];
+  T* kxl = reinterpret_cast<T*>(tws + L);
+  for (int i = tid; i < L; i += blockDim.x) {
+    tws[i] = tw[i];
+    kxl[i] = kxv[i];
+  }
   __syncthreads();
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -880,13 +890,12 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
     fft<T, L, -1>(nv[f], lds, t, idx, (const cx<T>*)tws);
   }
   if (!active) return;
-  const T ky = kyv[iy], kz = kzv[iz];
   const int my = mode_of(iy, g.ny), mz = mode_of(iz, g.nz);
   const T a = (T)rc.a, bb = (T)rc.b, dt = (T)rc.dt;
   const T nu = (T)rc.nu[f0];
   // right-hand side of element e: the projected nonlinear term (velocity) or its negative (scalar)
   auto rhs_of = [&](int e, cx<T>* rhs) {
-    const T kx = kxv[t + e * TPC];
+    const T kx = kxl[t + e * TPC];
     const T k2 = kx * kx + ky * ky + kz * kz;
     if (k2 == (T)0) {
 #pragma unroll
@@ -960,7 +969,7 @@ k_xfwd_rk(Geom g, CPtrs<T> q, CPtrs<T> dq, CPtrs<T> nl, const T* __restrict__ kx
         }
         continue;
       }
-      const T kx = kxv[p];
+      const T kx = kxl[p];
       const T k2 = kx * kx + ky * ky + kz * kz;
       cx<T> rhs[NF];
       if constexpr (RL) {
